@@ -1,0 +1,158 @@
+"""CPU FP64 oracle of the per-(E,k) layer kernel -- TEST INFRASTRUCTURE.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may use this package, and only as the checker.  The product
+path (``skkr_budapest_amd``) never imports it.
+
+``liboracle.so`` is built from ``skkr_oracle.c`` by ``make -C oracle``; each C
+routine cites the reference subroutine it restates.  This module wraps it with
+numpy arrays indexed ``A[row, col]``.
+
+Parity status of each routine (what pins it):
+  gjinv, surfgfd, dugo, tau2d, packer, k-loop/BZ sum -- golden vectors from
+      the reference's own Fortran compiled from /root/reference (kpar and
+      dbogo twins, tests/golden), plus dense-inverse and closed-form checks.
+  fourier -- no reference routine; brute force, k=0, Hermiticity,
+      reciprocal-lattice periodicity and linearity.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+STATUS = {0: "ok", 1: "singular pivot", 2: "decimation diverged",
+          3: "decimation itermax exceeded", 4: "out of memory"}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        i, d = ctypes.c_int, ctypes.c_double
+        L.orc_gjinv.argtypes = [P, i, i]
+        L.orc_surfgfd.argtypes = [P, P, P, P, i, i, i, d, P]
+        L.orc_dugo.argtypes = [P, P, P, P, i, i, ctypes.c_char, i, d, P, P]
+        L.orc_tau2d.argtypes = [P, P, P, P, i, i, i, P, P, d, P]
+        L.orc_fourier.argtypes = [i, P, P, i, P, P]
+        L.orc_pl_blocks.argtypes = [P, P, i, P, P, P]
+        L.orc_tau_bz.argtypes = [i, i, i, i, P, P, P, P, P, P, i, d, i, P, P, i,
+                                 P, P, P, P, P]
+        for f in (L.orc_gjinv, L.orc_surfgfd, L.orc_dugo, L.orc_tau2d, L.orc_tau_bz):
+            f.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _cm(a):
+    """[..., row, col] -> column-major complex128 buffer."""
+    return np.ascontiguousarray(np.swapaxes(np.asarray(a, np.complex128), -1, -2))
+
+
+def _uncm(b):
+    return np.ascontiguousarray(np.swapaxes(b, -1, -2))
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def gjinv(a):
+    b = _cm(a)
+    st = lib().orc_gjinv(_p(b), a.shape[0], a.shape[0])
+    return _uncm(b), st
+
+
+def surfgfd(ml, m0, mr, itermax=50, errmax=1e-8):
+    n = m0.shape[0]
+    bl, b0, br = _cm(ml), _cm(m0), _cm(mr)
+    x = np.zeros_like(b0)
+    it = ctypes.c_int(0)
+    st = lib().orc_surfgfd(_p(bl), _p(b0), _p(br), _p(x), n, n, itermax, errmax,
+                           ctypes.byref(it))
+    return _uncm(x), it.value, st
+
+
+def pl_blocks(g_intra_k, g_inter_k):
+    n = g_intra_k.shape[0]
+    gi, go = _cm(g_intra_k), _cm(g_inter_k)
+    m00, m01, m10 = (np.zeros_like(gi) for _ in range(3))
+    lib().orc_pl_blocks(_p(gi), _p(go), n, _p(m00), _p(m01), _p(m10))
+    return _uncm(m00), _uncm(m01), _uncm(m10)
+
+
+def dugo(m00g, m01, m10, tinv_lead, side, itermax=50, errmax=1e-8):
+    s, lk = tinv_lead.shape[0], tinv_lead.shape[1]
+    a, b, c, t = _cm(m00g), _cm(m01), _cm(m10), _cm(tinv_lead)
+    xy = np.zeros_like(a)
+    it = ctypes.c_int(0)
+    st = lib().orc_dugo(_p(a), _p(b), _p(c), _p(t), lk, s, side.encode(), itermax,
+                        errmax, _p(xy), ctypes.byref(it))
+    return _uncm(xy), it.value, st
+
+
+def tau2d(m00g, m01, m10, tinv, P, x, y, w=1.0):
+    lk = tinv.shape[1]
+    s = tinv.shape[0] // P
+    a, b, c, t = _cm(m00g), _cm(m01), _cm(m10), _cm(tinv)
+    xb, yb = _cm(x), _cm(y)
+    tau = np.zeros((P * s, lk, lk), np.complex128)
+    st = lib().orc_tau2d(_p(a), _p(b), _p(c), _p(t), lk, s, P, _p(xb), _p(yb),
+                         float(w), _p(tau))
+    return _uncm(tau), st
+
+
+def fourier(R, GR, k):
+    n = GR.shape[1]
+    Rb = np.ascontiguousarray(R, np.float64)
+    gb = _cm(GR)
+    kb = np.ascontiguousarray(k, np.float64)
+    out = np.zeros((n, n), np.complex128)
+    lib().orc_fourier(len(Rb), _p(Rb), _p(gb), n, _p(kb), _p(out))
+    return _uncm(out)
+
+
+def tau_bz(system, E, k, w, nworkers=1, want_tau_k=False, itermax=None,
+           errmax=None):
+    """The whole k-loop of one energy.  Returns dict with tau_bz (sites,lk,lk),
+    trk (nk, sites) traces of the unweighted per-k tau, iters (nk, 2) and
+    status (nk,), plus tau_k (nk, sites, lk, lk) when asked."""
+    cfg = system.cfg
+    lk, s, P = cfg.lk, cfg.s, cfg.P
+    tint, tl, tr = system.tinv(E)
+    R = np.ascontiguousarray(system.R, np.float64)
+    gi, go = _cm(system.G_intra), _cm(system.G_inter)
+    ti, tlb, trb = _cm(tint), _cm(tl), _cm(tr)
+    k = np.ascontiguousarray(k, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    nk = len(w)
+    ns = P * s
+    tau = np.zeros((ns, lk, lk), np.complex128)
+    tau_k = np.zeros((nk, ns, lk, lk), np.complex128) if want_tau_k else None
+    trk = np.zeros((nk, ns), np.complex128)
+    iters = np.zeros((nk, 2), np.int32)
+    status = np.zeros(nk, np.int32)
+    st = lib().orc_tau_bz(lk, s, P, len(R), _p(R), _p(gi), _p(go), _p(ti), _p(tlb),
+                          _p(trb), int(itermax or cfg.itermax),
+                          float(errmax or cfg.errmax), nk, _p(k), _p(w),
+                          int(nworkers), _p(tau),
+                          _p(tau_k) if tau_k is not None else None, _p(trk),
+                          _p(iters), _p(status))
+    out = {"tau_bz": _uncm(tau), "trk": trk, "iters": iters, "status": status,
+           "rc": st}
+    if tau_k is not None:
+        out["tau_k"] = _uncm(tau_k)
+    return out

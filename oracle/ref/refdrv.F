@@ -1,0 +1,154 @@
+c  Driver of the reference's per-(E,k) subroutines, compiled together with
+c  the reference sources where they lie (dugo, surfgfd, packer, tau2d,
+c  gjinv, matops of skkr_kpar_v6/main or skkr_dbogo_bsf/main) by
+c  oracle/ref/Makefile.  TEST INFRASTRUCTURE: it writes golden vectors.
+c
+c  It plays the role of the k-loop body of
+c  skkr_kpar_v6/rsp-scf/cpacoord_ksave.f:188-391 for a bulk-like stack
+c  (bulkgeo=.true.): per k it scatters G(k) into the layer-indexed
+c  g2d(lm,lm',jj,icount,icpl) layout that packer.f reads, calls
+c  dugo('L'), dugo('R') and tau2d, and accumulates w_k*tau.
+c
+c  Input (stream, little endian): int32 lk,s,P,nk,itermax,nsave;
+c  real*8 errmax; int32 isave(nsave); complex*16 tinv(lk,lk,P*s),
+c  tinvl(lk,lk,s), tinvr(lk,lk,s); real*8 w(nk); then per k
+c  complex*16 gintra(n,n), ginter(n,n) with n=lk*s (G, not -G).
+c  Output: complex*16 taubz(lk,lk,P*s), trk(P*s,nk), tausave(lk,lk,P*s,nsave).
+c  Decimation iterations are printed by surfgfd (ichk=1) on stdout.
+      program refdrv
+      implicit real*8 (a-h,o-z)
+      include 'param.h'
+#ifdef DBOGO
+      parameter(mdim=mdimdbogo, lgd=bogomaxp)
+#else
+      parameter(mdim=mdimr, lgd=lmmaxp)
+#endif
+      complex*16 g2d(lgd,lgd,-minprc:minprc,minprc,0:mprc1+1)
+      complex*16 xx(mdim,mdim),yy(mdim,mdim)
+      complex*16, allocatable :: tinv(:,:,:),tinvl(:,:,:),tinvr(:,:,:)
+      complex*16, allocatable :: tau1(:,:,:),taubz(:,:,:)
+      complex*16, allocatable :: trk(:,:),tsave(:,:,:,:)
+      complex*16, allocatable :: gi(:,:),go(:,:)
+      real*8, allocatable :: w(:)
+      integer, allocatable :: isave(:)
+      integer lk,ns,np,nk,itmx,nsave
+      real*8 errm
+      logical sgfconv
+      character*512 fin,fout
+#ifndef DBOGO
+      complex*16 fac
+      common/relfac/fac
+#endif
+      common/test/itest
+      common/iterpar/errmax,itermaxl,itermaxr,ichk
+      common/lay2d/cvec(mtotal,3),nextra,nbulkl,nbulkr,
+     &             nprc,ninprc(0:mprc+1)
+c
+      call get_command_argument(1,fin)
+      call get_command_argument(2,fout)
+      open(11,file=fin,access='stream',form='unformatted',status='old')
+      read(11) lk,ns,np,nk,itmx,nsave
+      read(11) errm
+      n=lk*ns
+      if(lk.gt.lgd.or.ns.gt.minprc.or.np.gt.mprc.or.n.gt.mdim) then
+        write(6,*) 'REFDRV: size exceeds param.h',lk,ns,np,n
+        stop 2
+      end if
+      allocate(isave(max(nsave,1)))
+      if(nsave.gt.0) read(11) isave(1:nsave)
+      allocate(tinv(lk,lk,mintfc),tinvl(lk,lk,minprc),
+     &         tinvr(lk,lk,minprc))
+      allocate(tau1(lk,lk,mintfc),taubz(lk,lk,np*ns))
+      allocate(trk(np*ns,nk),tsave(lk,lk,np*ns,max(nsave,1)))
+      allocate(gi(n,n),go(n,n),w(nk))
+      tinv=(0.d0,0.d0)
+      tinvl=(0.d0,0.d0)
+      tinvr=(0.d0,0.d0)
+      read(11) tinv(:,:,1:np*ns)
+      read(11) tinvl(:,:,1:ns)
+      read(11) tinvr(:,:,1:ns)
+      read(11) w
+c
+      itest=0
+      errmax=errm
+      itermaxl=itmx
+      itermaxr=itmx
+      ichk=1
+      nprc=np
+      do i=0,mprc+1
+        ninprc(i)=ns
+      end do
+#ifndef DBOGO
+      fac=(1.d0,0.d0)
+#endif
+      taubz=(0.d0,0.d0)
+      lmax=2
+c
+      do ik=1,nk
+        read(11) gi
+        read(11) go
+        g2d=(0.d0,0.d0)
+        do ipl=0,2
+        do i=1,ns
+c         coupling to PL p-1: offsets -s..-i, column site jp=s+i+j
+          do j=-ns,-i
+            jp=ns+i+j
+            do ia=1,lk
+            do ib=1,lk
+              g2d(ia,ib,j,i,ipl)=dconjg(go((jp-1)*lk+ib,(i-1)*lk+ia))
+            end do
+            end do
+          end do
+c         within the PL: offsets 1-i..s-i, column site jp=i+j
+          do j=1-i,ns-i
+            jp=i+j
+            do ia=1,lk
+            do ib=1,lk
+              g2d(ia,ib,j,i,ipl)=gi((i-1)*lk+ia,(jp-1)*lk+ib)
+            end do
+            end do
+          end do
+c         coupling to PL p+1: offsets s-i+1..s, column site jp=j+i-s
+          do j=ns-i+1,ns
+            jp=j+i-ns
+            do ia=1,lk
+            do ib=1,lk
+              g2d(ia,ib,j,i,ipl)=go((i-1)*lk+ia,(jp-1)*lk+ib)
+            end do
+            end do
+          end do
+        end do
+        end do
+c
+        call dugo(lmax,lk,tinvl,lk,g2d(1,1,-minprc,1,0),
+     &            g2d(1,1,-minprc,1,1),yy,mdim,'L',.true.,0,sgfconv)
+        if(.not.sgfconv) stop 3
+        call dugo(lmax,lk,tinvr,lk,g2d(1,1,-minprc,1,2),
+     &            g2d(1,1,-minprc,1,1),xx,mdim,'R',.true.,0,sgfconv)
+        if(.not.sgfconv) stop 3
+        call tau2d(0,1.d0,.true.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
+     &             tau1,mdim)
+        do li=1,np*ns
+          trk(li,ik)=(0.d0,0.d0)
+          do l=1,lk
+            trk(li,ik)=trk(li,ik)+tau1(l,l,li)
+          end do
+          do l=1,lk
+          do lp=1,lk
+            taubz(l,lp,li)=taubz(l,lp,li)+w(ik)*tau1(l,lp,li)
+          end do
+          end do
+        end do
+        do is=1,nsave
+          if(isave(is).eq.ik) tsave(:,:,:,is)=tau1(:,:,1:np*ns)
+        end do
+      end do
+      close(11)
+c
+      open(12,file=fout,access='stream',form='unformatted',
+     &     status='replace')
+      write(12) taubz
+      write(12) trk
+      if(nsave.gt.0) write(12) tsave(:,:,:,1:nsave)
+      close(12)
+      end

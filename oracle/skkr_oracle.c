@@ -1,0 +1,445 @@
+/*
+ * CPU FP64 oracle of the per-(E,k) layer kernel.  TEST INFRASTRUCTURE ONLY:
+ * it is the checker of the HIP path (tests, the smoke check and the CPU
+ * baseline leg of the benchmark) and is never part of the product.
+ *
+ * Each routine restates one reference subroutine, keeping its operation
+ * order (loop nesting, summation order, pivot rule) so that results agree
+ * with the reference to round-off:
+ *
+ *   orc_gjinv    skkr_kpar_v6/main/gjinv.f:2-59     Gauss-Jordan, column pivot
+ *   orc_doubmt2  skkr_kpar_v6/main/matops.f:204-228 c = a*b
+ *   orc_tripmt   skkr_kpar_v6/main/matops.f:294-342 b = u*b*ust
+ *   orc_tripmt2  skkr_kpar_v6/main/matops.f:393-444 b1 = u*b*v
+ *   orc_surfgfd  skkr_kpar_v6/main/surfgf.f:2-108   decimation
+ *   orc_dugo     skkr_kpar_v6/main/dugo.f:9-120     lead self-energy (bulk)
+ *   orc_tau2d    skkr_kpar_v6/main/tau2d.f:2-152    Godfrin sweeps
+ *   orc_fourier  G(k) = sum_R G_r(R) exp(-i k.R)    (no reference routine;
+ *                pinned by brute force and closed forms in the tests)
+ *   orc_tau_bz   k-loop body of skkr_kpar_v6/rsp-scf/cpacoord_ksave.f:188-391
+ *                (weights inside tau2d, BZ sum with addmat, round-robin split
+ *                over workers like modulo(ik-1,nprocs), partial sums added
+ *                in worker order like the MPI_Allreduce at :406-407)
+ *
+ * The packer (skkr_kpar_v6/main/packer.f:6-100) for equal sites per PL is
+ * the assembly M00 = -G_intra(k), M01 = -G_inter(k), M10 = M01^dagger done by
+ * orc_pl_blocks; its layer-window logic is already in the zero pattern of
+ * G_inter (site j of PL p+1 couples to site i of PL p only if j <= i).
+ *
+ * All matrices are column-major double complex with an explicit leading
+ * dimension, as complex*16 arrays are in the reference.
+ */
+#include <complex.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double _Complex cplx;
+#define AT(a, i, j, ld) ((a)[(size_t)(i) + (size_t)(j) * (size_t)(ld)])
+
+enum {
+  ORC_OK = 0,
+  ORC_SINGULAR = 1,      /* zero pivot met in gjinv */
+  ORC_SGF_DIVERGED = 2,  /* surfgfd: |alpha| above 1e10 (sgfconv=.false.) */
+  ORC_SGF_ITERMAX = 3,   /* surfgfd: iter > itermax (reference stops) */
+  ORC_NOMEM = 4
+};
+
+/* gjinv.f:2-59.  In-place inverse; column pivot search along row i
+ * (:13-22), column swap (:27-30), scaling of column i (:36-38),
+ * elimination of every other column (:40-46), row unscramble (:49-56). */
+int orc_gjinv(cplx *a, int n, int lda) {
+  int *ind = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int status = ORC_OK;
+  if (!ind) return ORC_NOMEM;
+  for (int i = 0; i < n; ++i) {
+    int k = i;
+    double amaxi = cabs(AT(a, i, i, lda));
+    for (int j = i + 1; j < n; ++j) {
+      double atest = cabs(AT(a, i, j, lda));
+      if (atest <= amaxi) continue;
+      k = j;
+      amaxi = atest;
+    }
+    ind[i] = k;
+    if (k != i) {
+      for (int j = 0; j < n; ++j) {
+        cplx q = AT(a, j, i, lda);
+        AT(a, j, i, lda) = AT(a, j, k, lda);
+        AT(a, j, k, lda) = q;
+      }
+    }
+    cplx q = AT(a, i, i, lda);
+    if (q == 0.0) status = ORC_SINGULAR;
+    q = 1.0 / q;
+    AT(a, i, i, lda) = 1.0;
+    for (int j = 0; j < n; ++j) AT(a, j, i, lda) = AT(a, j, i, lda) * q;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      q = AT(a, i, j, lda);
+      AT(a, i, j, lda) = 0.0;
+      for (int kk = 0; kk < n; ++kk)
+        AT(a, kk, j, lda) = AT(a, kk, j, lda) - AT(a, kk, i, lda) * q;
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    int j = ind[i];
+    if (i == j) continue;
+    for (int k = 0; k < n; ++k) {
+      cplx q = AT(a, i, k, lda);
+      AT(a, i, k, lda) = AT(a, j, k, lda);
+      AT(a, j, k, lda) = q;
+    }
+  }
+  free(ind);
+  return status;
+}
+
+/* matops.f:204-228: c(n1,n3) = a(n1,n2) * b(n2,n3), j summed in order. */
+void orc_doubmt2(const cplx *a, const cplx *b, cplx *c, int n1, int n2, int n3,
+                 int ld) {
+  for (int i = 0; i < n1; ++i)
+    for (int k = 0; k < n3; ++k) {
+      cplx sum = 0.0;
+      for (int j = 0; j < n2; ++j) sum = sum + AT(b, j, k, ld) * AT(a, i, j, ld);
+      AT(c, i, k, ld) = sum;
+    }
+}
+
+/* matops.f:294-342: b(n1,n2) = u(n1,n1) * b * ust(n2,n2). */
+void orc_tripmt(const cplx *u, cplx *b, const cplx *ust, int n1, int n2,
+                int ld) {
+  cplx *c = (cplx *)malloc(sizeof(cplx) * (size_t)n1 * (size_t)n2);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n2; ++j) {
+      cplx x = 0.0;
+      for (int k = 0; k < n1; ++k) x = x + AT(b, k, j, ld) * AT(u, i, k, ld);
+      c[i + (size_t)j * n1] = x;
+    }
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n2; ++j) {
+      cplx x = 0.0;
+      for (int k = 0; k < n2; ++k) x = x + AT(ust, k, j, ld) * c[i + (size_t)k * n1];
+      AT(b, i, j, ld) = x;
+    }
+  free(c);
+}
+
+/* matops.f:393-444: b1(n1,n4) = u(n1,n2) * b(n2,n3) * v(n3,n4). */
+void orc_tripmt2(const cplx *u, const cplx *b, const cplx *v, cplx *b1, int n1,
+                 int n2, int n3, int n4, int ld, int ld1) {
+  cplx *c = (cplx *)malloc(sizeof(cplx) * (size_t)n1 * (size_t)n3);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n3; ++j) {
+      cplx x = 0.0;
+      for (int k = 0; k < n2; ++k) x = x + AT(b, k, j, ld) * AT(u, i, k, ld);
+      c[i + (size_t)j * n1] = x;
+    }
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n4; ++j) {
+      cplx x = 0.0;
+      for (int k = 0; k < n3; ++k) x = x + AT(v, k, j, ld) * c[i + (size_t)k * n1];
+      AT(b1, i, j, ld1) = x;
+    }
+  free(c);
+}
+
+/* surfgf.f:2-108.  Decimation for the surface of a semi-infinite chain whose
+ * surface block couples to the chain through ml (and back through mr):
+ * eps=m0, alpha=-ml, beta=-mr, x=m0 (:38-45); per iteration y=eps^-1,
+ * alpha<-a y a, beta<-b y b, eps<-eps - a y b - b y a, x<-x - a y b (:47-75);
+ * err = |alpha|_F after the update (:76-81); stop when err<errmax or
+ * iter>itermax (:88); err>1e10 flags divergence (:89-94); iter>itermax is
+ * fatal (:100-103); returns x^-1 (:105).  *iters = iterations done. */
+int orc_surfgfd(const cplx *ml, const cplx *m0, const cplx *mr, cplx *x, int n,
+                int ld, int itermax, double errmax, int *iters) {
+  size_t nn = (size_t)ld * (size_t)n;
+  cplx *alfa = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *beta = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *eps = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *y1 = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *y2 = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *y3 = (cplx *)calloc(nn, sizeof(cplx));
+  cplx *y4 = (cplx *)calloc(nn, sizeof(cplx));
+  int status = ORC_OK;
+  if (!alfa || !beta || !eps || !y1 || !y2 || !y3 || !y4) {
+    status = ORC_NOMEM;
+    goto done;
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      AT(eps, i, j, ld) = AT(m0, i, j, ld);
+      AT(alfa, i, j, ld) = -AT(ml, i, j, ld);
+      AT(beta, i, j, ld) = -AT(mr, i, j, ld);
+      AT(x, i, j, ld) = AT(m0, i, j, ld);
+    }
+  int iter = 1;
+  for (;;) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) AT(y1, i, j, ld) = AT(eps, i, j, ld);
+    if (orc_gjinv(y1, n, ld) != ORC_OK) status = ORC_SINGULAR;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        AT(y2, i, j, ld) = AT(y1, i, j, ld);
+        AT(y3, i, j, ld) = AT(y1, i, j, ld);
+        AT(y4, i, j, ld) = AT(y1, i, j, ld);
+      }
+    orc_tripmt(alfa, y1, alfa, n, n, ld);
+    orc_tripmt(beta, y2, beta, n, n, ld);
+    orc_tripmt(alfa, y3, beta, n, n, ld);
+    orc_tripmt(beta, y4, alfa, n, n, ld);
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        AT(alfa, i, j, ld) = AT(y1, i, j, ld);
+        AT(beta, i, j, ld) = AT(y2, i, j, ld);
+        AT(eps, i, j, ld) = AT(eps, i, j, ld) - AT(y3, i, j, ld) - AT(y4, i, j, ld);
+        AT(x, i, j, ld) = AT(x, i, j, ld) - AT(y3, i, j, ld);
+        double xre = creal(AT(alfa, i, j, ld)), xim = cimag(AT(alfa, i, j, ld));
+        sum = sum + xre * xre + xim * xim;
+      }
+    double err = sqrt(sum);
+    if (err < errmax || iter > itermax) break;
+    if (err > 1.0e10) {
+      if (status == ORC_OK) status = ORC_SGF_DIVERGED;
+      break;
+    }
+    iter = iter + 1;
+  }
+  *iters = iter;
+  if (iter > itermax && status == ORC_OK) status = ORC_SGF_ITERMAX;
+  if (orc_gjinv(x, n, ld) != ORC_OK && status == ORC_OK) status = ORC_SINGULAR;
+done:
+  free(alfa); free(beta); free(eps); free(y1); free(y2); free(y3); free(y4);
+  return status;
+}
+
+/* Add block-diagonal site t^-1 (lk x lk per site, s sites) to m:
+ * m00(i,j) = tminv(k,k',site) + m00(i,j)  (dugo.f:82-90, tau2d.f:63-70). */
+static void add_tinv(cplx *m, int ld, const cplx *tinv, int lk, int s) {
+  for (int a = 0; a < s; ++a)
+    for (int k1 = 0; k1 < lk; ++k1)
+      for (int k2 = 0; k2 < lk; ++k2) {
+        int i = a * lk + k1, j = a * lk + k2;
+        AT(m, i, j, ld) = tinv[(size_t)a * lk * lk + k1 + (size_t)k2 * lk] + AT(m, i, j, ld);
+      }
+}
+
+/* dugo.f:9-120, bulk branch (the drivers always pass bulk=.true.,
+ * cpacoord.f:219-229,252-262).  m00g = -G_intra(k), m01 = -G_inter(k),
+ * m10 = m01^dagger (n x n, leading dim n).  side 'L': surfgfd(m10,m00,m01)
+ * then xy = m10 * g_s * m01 (:95-98); side 'R': surfgfd(m01,m00,m10) then
+ * xy = m01 * g_s * m10 (:103-106). */
+int orc_dugo(const cplx *m00g, const cplx *m01, const cplx *m10,
+             const cplx *tinv_lead, int lk, int s, char side, int itermax,
+             double errmax, cplx *xy, int *iters) {
+  int n = lk * s;
+  size_t nn = (size_t)n * n;
+  cplx *m00 = (cplx *)malloc(sizeof(cplx) * nn);
+  cplx *gs = (cplx *)malloc(sizeof(cplx) * nn);
+  if (!m00 || !gs) { free(m00); free(gs); return ORC_NOMEM; }
+  memcpy(m00, m00g, sizeof(cplx) * nn);
+  add_tinv(m00, n, tinv_lead, lk, s);
+  int st;
+  if (side == 'L') {
+    st = orc_surfgfd(m10, m00, m01, gs, n, n, itermax, errmax, iters);
+    orc_tripmt2(m10, gs, m01, xy, n, n, n, n, n, n);
+  } else {
+    st = orc_surfgfd(m01, m00, m10, gs, n, n, itermax, errmax, iters);
+    orc_tripmt2(m01, gs, m10, xy, n, n, n, n, n, n);
+  }
+  free(m00); free(gs);
+  return st;
+}
+
+/* tau2d.f:2-152 with bulkgeo (same G blocks for every PL).
+ * tinv: P*s site blocks; x = X_P (right lead), y = Y_1 (left lead);
+ * tau: P*s site blocks, each multiplied by w (:137). */
+int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
+              const cplx *tinv, int lk, int s, int P, const cplx *x,
+              const cplx *y, double w, cplx *tau) {
+  int n = lk * s;
+  size_t nn = (size_t)n * n;
+  cplx *xx = (cplx *)malloc(sizeof(cplx) * nn * P);
+  cplx *yy = (cplx *)malloc(sizeof(cplx) * nn * P);
+  cplx *t00 = (cplx *)malloc(sizeof(cplx) * nn * P);
+  cplx *m00 = (cplx *)malloc(sizeof(cplx) * nn);
+  cplx *help = (cplx *)malloc(sizeof(cplx) * nn);
+  int status = ORC_OK;
+  if (!xx || !yy || !t00 || !m00 || !help) { status = ORC_NOMEM; goto done; }
+  memcpy(yy, y, sizeof(cplx) * nn);
+  memcpy(xx + nn * (P - 1), x, sizeof(cplx) * nn);
+  /* Y sweep (:53-85) */
+  for (int p = 0; p < P; ++p) {
+    memcpy(m00, m00g, sizeof(cplx) * nn);
+    add_tinv(m00, n, tinv + (size_t)p * s * lk * lk, lk, s);
+    memcpy(t00 + nn * p, m00, sizeof(cplx) * nn);
+    if (p > 0) {
+      orc_doubmt2(m10, yy + nn * p, help, n, n, n, n);
+      memcpy(yy + nn * p, help, sizeof(cplx) * nn);
+    }
+    if (p < P - 1) {
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          AT(m00, i, j, n) = AT(m00, i, j, n) - AT(yy + nn * p, i, j, n);
+      if (orc_gjinv(m00, n, n) != ORC_OK) status = ORC_SINGULAR;
+      orc_doubmt2(m00, m01, yy + nn * (p + 1), n, n, n, n);
+    }
+  }
+  /* X sweep (:89-109) */
+  for (int p = P - 1; p >= 0; --p) {
+    if (p < P - 1) {
+      orc_doubmt2(m01, xx + nn * p, m00, n, n, n, n);
+      memcpy(xx + nn * p, m00, sizeof(cplx) * nn);
+    }
+    if (p > 0) {
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          AT(m00, i, j, n) = AT(t00 + nn * p, i, j, n) - AT(xx + nn * p, i, j, n);
+      if (orc_gjinv(m00, n, n) != ORC_OK) status = ORC_SINGULAR;
+      orc_doubmt2(m00, m10, xx + nn * (p - 1), n, n, n, n);
+    }
+  }
+  /* diagonal blocks (:114-149) */
+  for (int p = 0; p < P; ++p) {
+    cplx *t = t00 + nn * p;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j)
+        AT(t, i, j, n) = AT(t, i, j, n) - AT(xx + nn * p, i, j, n) - AT(yy + nn * p, i, j, n);
+    if (orc_gjinv(t, n, n) != ORC_OK) status = ORC_SINGULAR;
+    for (int a = 0; a < s; ++a) {
+      cplx *ts = tau + ((size_t)p * s + a) * lk * lk;
+      for (int l = 0; l < lk; ++l)
+        for (int lp = 0; lp < lk; ++lp)
+          ts[l + (size_t)lp * lk] = w * AT(t, a * lk + l, a * lk + lp, n);
+    }
+  }
+done:
+  free(xx); free(yy); free(t00); free(m00); free(help);
+  return status;
+}
+
+/* G(k) = sum_R G_r(R) exp(-i k.R), R summed in index order. */
+void orc_fourier(int nR, const double *R, const cplx *GR, int n, const double *k,
+                 cplx *out) {
+  size_t nn = (size_t)n * n;
+  for (size_t e = 0; e < nn; ++e) out[e] = 0.0;
+  for (int r = 0; r < nR; ++r) {
+    double ph = k[0] * R[2 * r] + k[1] * R[2 * r + 1];
+    cplx f = cos(ph) - I * sin(ph);
+    const cplx *g = GR + nn * r;
+    for (size_t e = 0; e < nn; ++e) out[e] = out[e] + g[e] * f;
+  }
+}
+
+/* packer.f:6-100 for equal sites per PL: M00 = -G_intra, M01 = -G_inter,
+ * M10 = -G_inter^dagger. */
+void orc_pl_blocks(const cplx *gintra, const cplx *ginter, int n, cplx *m00,
+                   cplx *m01, cplx *m10) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      AT(m00, i, j, n) = -AT(gintra, i, j, n);
+      AT(m01, i, j, n) = -AT(ginter, i, j, n);
+      AT(m10, i, j, n) = -conj(AT(ginter, j, i, n));
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* k-loop                                                              */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int lk, s, P, nR, itermax;
+  double errmax;
+  const double *R;
+  const cplx *GRi, *GRo, *tint, *tl, *tr;
+  int nk;
+  const double *k, *w;
+  int rank, nprocs;
+  cplx *tau_part;   /* P*s*lk*lk, accumulated */
+  cplx *tau_k;      /* optional: nk * P*s*lk*lk unweighted tau per k */
+  cplx *trk;        /* optional: nk * P*s traces of unweighted tau */
+  int *iters;       /* optional: nk * 2 (left, right) */
+  int *status_k;    /* optional: nk */
+  int status;
+} kloop_arg;
+
+static void *kloop_worker(void *p) {
+  kloop_arg *a = (kloop_arg *)p;
+  int n = a->lk * a->s, lk = a->lk, ns = a->P * a->s;
+  size_t nn = (size_t)n * n, tsz = (size_t)ns * lk * lk;
+  cplx *gi = malloc(sizeof(cplx) * nn), *go = malloc(sizeof(cplx) * nn);
+  cplx *m00 = malloc(sizeof(cplx) * nn), *m01 = malloc(sizeof(cplx) * nn);
+  cplx *m10 = malloc(sizeof(cplx) * nn), *xx = malloc(sizeof(cplx) * nn);
+  cplx *yy = malloc(sizeof(cplx) * nn), *tau1 = malloc(sizeof(cplx) * tsz);
+  a->status = ORC_OK;
+  if (!gi || !go || !m00 || !m01 || !m10 || !xx || !yy || !tau1) {
+    a->status = ORC_NOMEM;
+    goto done;
+  }
+  for (int ik = 0; ik < a->nk; ++ik) {
+    if (ik % a->nprocs != a->rank) continue;
+    const double *kk = a->k + 2 * ik;
+    orc_fourier(a->nR, a->R, a->GRi, n, kk, gi);
+    orc_fourier(a->nR, a->R, a->GRo, n, kk, go);
+    orc_pl_blocks(gi, go, n, m00, m01, m10);
+    int itl = 0, itr = 0;
+    int st = orc_dugo(m00, m01, m10, a->tl, lk, a->s, 'L', a->itermax, a->errmax, yy, &itl);
+    int st2 = orc_dugo(m00, m01, m10, a->tr, lk, a->s, 'R', a->itermax, a->errmax, xx, &itr);
+    if (st == ORC_OK) st = st2;
+    int st3 = orc_tau2d(m00, m01, m10, a->tint, lk, a->s, a->P, xx, yy, 1.0, tau1);
+    if (st == ORC_OK) st = st3;
+    if (a->iters) { a->iters[2 * ik] = itl; a->iters[2 * ik + 1] = itr; }
+    if (a->status_k) a->status_k[ik] = st;
+    if (st != ORC_OK && a->status == ORC_OK) a->status = st;
+    double wk = a->w[ik];
+    for (size_t e = 0; e < tsz; ++e) a->tau_part[e] = a->tau_part[e] + wk * tau1[e];
+    if (a->tau_k) memcpy(a->tau_k + tsz * ik, tau1, sizeof(cplx) * tsz);
+    if (a->trk)
+      for (int site = 0; site < ns; ++site) {
+        cplx tr = 0.0;
+        for (int l = 0; l < lk; ++l) tr = tr + tau1[(size_t)site * lk * lk + l + (size_t)l * lk];
+        a->trk[(size_t)ik * ns + site] = tr;
+      }
+  }
+done:
+  free(gi); free(go); free(m00); free(m01); free(m10); free(xx); free(yy); free(tau1);
+  return NULL;
+}
+
+/* The k-loop: tau_bz += sum_k w_k tau(k) over site-diagonal blocks.
+ * nworkers threads take k round-robin (modulo(ik-1,nprocs)==myrank) and their
+ * partial sums are added in worker order. */
+int orc_tau_bz(int lk, int s, int P, int nR, const double *R, const cplx *GRi,
+               const cplx *GRo, const cplx *tint, const cplx *tl, const cplx *tr,
+               int itermax, double errmax, int nk, const double *k,
+               const double *w, int nworkers, cplx *tau_bz, cplx *tau_k,
+               cplx *trk, int *iters, int *status_k) {
+  if (nworkers < 1) nworkers = 1;
+  size_t tsz = (size_t)P * s * lk * lk;
+  kloop_arg *args = calloc((size_t)nworkers, sizeof(kloop_arg));
+  pthread_t *th = calloc((size_t)nworkers, sizeof(pthread_t));
+  if (!args || !th) { free(args); free(th); return ORC_NOMEM; }
+  for (int r = 0; r < nworkers; ++r) {
+    kloop_arg *a = &args[r];
+    a->lk = lk; a->s = s; a->P = P; a->nR = nR; a->itermax = itermax;
+    a->errmax = errmax; a->R = R; a->GRi = GRi; a->GRo = GRo; a->tint = tint;
+    a->tl = tl; a->tr = tr; a->nk = nk; a->k = k; a->w = w; a->rank = r;
+    a->nprocs = nworkers; a->tau_k = tau_k; a->trk = trk; a->iters = iters;
+    a->status_k = status_k;
+    a->tau_part = calloc(tsz, sizeof(cplx));
+    if (nworkers == 1) kloop_worker(a);
+    else pthread_create(&th[r], NULL, kloop_worker, a);
+  }
+  int status = ORC_OK;
+  for (int r = 0; r < nworkers; ++r) {
+    if (nworkers > 1) pthread_join(th[r], NULL);
+  }
+  for (int r = 0; r < nworkers; ++r) {
+    for (size_t e = 0; e < tsz; ++e) tau_bz[e] = tau_bz[e] + args[r].tau_part[e];
+    if (args[r].status != ORC_OK && status == ORC_OK) status = args[r].status;
+    free(args[r].tau_part);
+  }
+  free(args); free(th);
+  return status;
+}

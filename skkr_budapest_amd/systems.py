@@ -1,0 +1,317 @@
+"""Seeded synthetic layered systems for the per-(E,k) layer kernel.
+
+A system is the input of the k-loop of ``cpacoord``
+(skkr_kpar_v6/nrsp-scf/cpacoord.f:147-317): real-space structure constants
+G_r(R) for the in-plane vectors R (intra-principal-layer and
+nearest-principal-layer couplings), inverse single-site t-matrices of every
+interface site and of the two semi-infinite leads, the energies and the
+2D k-mesh with weights.
+
+Conventions (all fixed here and used by every consumer):
+
+* Site-major ordering inside a principal layer (PL): row/column index
+  ``kk = site*lk + orbital`` (skkr_kpar_v6/main/packer.f:47-95).
+* Lattice Fourier sum ``G(k) = sum_R G_r(R) exp(-i k.R)``.
+* Hermitian symmetrisation per R/-R pair: ``G_r(-R) = G_r(R)^dagger`` for the
+  intra-PL coupling; the coupling of PL p+1 back to PL p at R is
+  ``G_inter(-R)^dagger``, so ``M10(k) = M01(k)^dagger``.
+* Principal-layer window: site i of PL p couples to site j of PL p+1 only if
+  j <= i (the layer-offset window |j| <= s of
+  skkr_kpar_v6/main/gstore.f:255-273 and packer.f:36-46).
+* The matrix inverted per k is ``t^-1 - G(k)``; every PL and both leads share
+  G_r (``bulkgeo``, skkr_kpar_v6/main/tau2d.f:55-56).
+
+Matrices are held as numpy arrays indexed ``A[row, col]``; the C-ABI wants
+column-major buffers, which :func:`colmajor` produces.
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import math
+
+import numpy as np
+
+SEED_BASE = 20261203
+
+
+# --------------------------------------------------------------------------
+# lattice and meshes
+# --------------------------------------------------------------------------
+def r_vectors(r2max: int) -> np.ndarray:
+    """Integer in-plane vectors of the square lattice (a=1) with |R|^2 <= r2max,
+    sorted by |R| and then by polar angle in [0, 2pi)."""
+    m = int(math.isqrt(r2max)) + 1
+    pts = [(x, y) for x in range(-m, m + 1) for y in range(-m, m + 1)
+           if x * x + y * y <= r2max]
+
+    def key(p):
+        ang = math.atan2(p[1], p[0])
+        if ang < 0:
+            ang += 2 * math.pi
+        return (p[0] ** 2 + p[1] ** 2, round(ang, 12))
+
+    pts.sort(key=key)
+    return np.array(pts, dtype=np.float64)
+
+
+def uniform_kmesh(nside: int):
+    """Symmetric uniform 2D mesh k_i = 2pi((i+1/2)/N - 1/2) per axis, equal
+    weights normalised to 1 (skkr_kpar_v6/main/kmesh2d.f:245-251)."""
+    ax = 2.0 * np.pi * ((np.arange(nside) + 0.5) / nside - 0.5)
+    kx, ky = np.meshgrid(ax, ax, indexing="ij")
+    k = np.stack([kx.ravel(), ky.ravel()], axis=1)
+    w = np.full(k.shape[0], 1.0)
+    w /= w.sum()
+    return k, w
+
+
+def colmajor(a: np.ndarray) -> np.ndarray:
+    """Stack of matrices [..., row, col] -> C-contiguous buffer whose memory is
+    column-major per matrix (Fortran ``complex*16 a(n,n)`` layout)."""
+    return np.ascontiguousarray(np.swapaxes(a, -1, -2))
+
+
+def from_colmajor(buf: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.swapaxes(buf, -1, -2))
+
+
+# --------------------------------------------------------------------------
+# configurations
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class Config:
+    name: str
+    cfg_id: int
+    lk: int                 # site block width
+    s: int                  # sites per principal layer
+    P: int                  # interface principal layers
+    r2max: int              # in-plane vectors with |R|^2 <= r2max
+    tmodel: str             # "diag" | "herm" | "bdg"
+    spin_diag: bool = False # G spin-independent (nonrelativistic)
+    energies: tuple = ()
+    mesh: int = 16          # uniform mesh side for a single energy
+    delta: float = 1.0e-3   # BdG pairing amplitude (Ry)
+    errmax: float = 1.0e-8
+    itermax: int = 50
+
+    @property
+    def n(self) -> int:
+        return self.lk * self.s
+
+    @property
+    def n_sites(self) -> int:
+        return self.P * self.s
+
+    @property
+    def n_R(self) -> int:
+        return len(r_vectors(self.r2max))
+
+
+CONFIGS = {
+    0: Config("cfg0-nrsp-lmax2", 0, lk=18, s=2, P=8, r2max=4, tmodel="diag",
+              spin_diag=True, energies=(0.5 + 0.05j,), mesh=16),
+    1: Config("cfg1-rel-lmax3", 1, lk=32, s=4, P=20, r2max=8, tmodel="herm",
+              energies=(0.6 + 0.02j,), mesh=64),
+    2: Config("cfg2-rel-contour", 2, lk=32, s=4, P=20, r2max=8, tmodel="herm",
+              mesh=64),
+    3: Config("cfg3-bdg", 3, lk=64, s=4, P=12, r2max=8, tmodel="bdg"),
+    4: Config("cfg4-rel-adaptive", 4, lk=32, s=4, P=20, r2max=8, tmodel="herm",
+              energies=(0.6 + 0.005j,), mesh=32),
+}
+
+
+def small_config(lk=4, s=2, P=3, r2max=2, tmodel="herm", spin_diag=False,
+                 cfg_id=90, energies=(0.5 + 0.1j,), mesh=4, name=None):
+    """A reduced system with the generation law of the configs (tests)."""
+    return Config(name or f"small-lk{lk}-s{s}-P{P}", cfg_id, lk=lk, s=s, P=P,
+                  r2max=r2max, tmodel=tmodel, spin_diag=spin_diag,
+                  energies=tuple(energies), mesh=mesh)
+
+
+# --------------------------------------------------------------------------
+# random generation
+# --------------------------------------------------------------------------
+def _rng(cfg: Config, stream: int) -> np.random.Generator:
+    return np.random.default_rng(SEED_BASE + 1000 * cfg.cfg_id + stream)
+
+
+def _phases(rng, shape):
+    return np.exp(1j * rng.uniform(0.0, 2.0 * np.pi, size=shape))
+
+
+def _pair_index(R: np.ndarray) -> np.ndarray:
+    idx = {(int(x), int(y)): i for i, (x, y) in enumerate(R)}
+    return np.array([idx[(-int(x), -int(y))] for x, y in R])
+
+
+def _expand_spin(g: np.ndarray, s: int, norb: int) -> np.ndarray:
+    """(.., s*norb, s*norb) spin-independent blocks -> (.., s*2norb, s*2norb)
+    with site block [spin-up orbitals | spin-down orbitals]."""
+    lk = 2 * norb
+    out = np.zeros(g.shape[:-2] + (s * lk, s * lk), dtype=g.dtype)
+    for a in range(s):
+        for b in range(s):
+            blk = g[..., a * norb:(a + 1) * norb, b * norb:(b + 1) * norb]
+            for sp in range(2):
+                out[..., a * lk + sp * norb:a * lk + (sp + 1) * norb,
+                    b * lk + sp * norb:b * lk + (sp + 1) * norb] = blk
+    return out
+
+
+def _gen_structure(cfg: Config, lk_gen: int):
+    """Intra and inter G_r(R) with site blocks of width lk_gen."""
+    rng = _rng(cfg, 0)
+    R = r_vectors(cfg.r2max)
+    nR = len(R)
+    n = lk_gen * cfg.s
+    pair = _pair_index(R)
+    mag = np.exp(-2.0 * np.sqrt((R ** 2).sum(1)))
+    intra = np.zeros((nR, n, n), dtype=np.complex128)
+    for i in range(nR):
+        j = pair[i]
+        if j < i:
+            continue
+        if j == i:  # R = 0: Hermitian, unit-magnitude entries
+            a = _phases(rng, (n, n))
+            h = np.triu(a, 1)
+            d = np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0)
+            intra[i] = h + h.conj().T + np.diag(d)
+        else:
+            a = mag[i] * _phases(rng, (n, n))
+            intra[i] = a
+            intra[j] = a.conj().T
+    inter = mag[:, None, None] * _phases(rng, (nR, n, n))
+    # principal-layer window: site i of PL p couples to site j of PL p+1 iff j <= i
+    site = np.arange(n) // lk_gen
+    mask = site[None, :] <= site[:, None]
+    inter = inter * mask[None]
+    return R, intra, inter
+
+
+def _herm_site(rng, lk):
+    """Hermitian site Hamiltonian: diagonal U(-1,1), off-diagonal random phase
+    with each row's off-diagonal magnitude sum = 0.25 (diagonally dominant
+    against the E-shifted diagonal away from its zeros)."""
+    c = 0.25 / max(lk - 1, 1)
+    a = c * _phases(rng, (lk, lk))
+    h = np.triu(a, 1)
+    return h + h.conj().T + np.diag(rng.uniform(-1.0, 1.0, size=lk))
+
+
+def _gen_t0(cfg: Config, rng, nsites: int) -> np.ndarray:
+    lk = cfg.lk
+    t0 = np.zeros((nsites, lk, lk), dtype=np.complex128)
+    for a in range(nsites):
+        if cfg.tmodel == "diag":
+            d = rng.uniform(-1.0, 1.0, size=lk) + 1j * rng.uniform(0.05, 0.3, size=lk)
+            t0[a] = np.diag(d)
+        elif cfg.tmodel == "herm":
+            t0[a] = _herm_site(rng, lk)
+        elif cfg.tmodel == "bdg":
+            h = lk // 2
+            he = _herm_site(rng, h)
+            t0[a, :h, :h] = he
+            t0[a, h:, h:] = -he.conj()
+            t0[a, :h, h:] = cfg.delta * np.eye(h)
+            t0[a, h:, :h] = cfg.delta * np.eye(h)
+        else:
+            raise ValueError(cfg.tmodel)
+    return t0
+
+
+@dataclasses.dataclass
+class System:
+    cfg: Config
+    R: np.ndarray          # (n_R, 2)
+    G_intra: np.ndarray    # (n_R, n, n)
+    G_inter: np.ndarray    # (n_R, n, n)
+    t0_int: np.ndarray     # (P*s, lk, lk)
+    t0_left: np.ndarray    # (s, lk, lk)
+    t0_right: np.ndarray   # (s, lk, lk)
+
+    @property
+    def lk(self):
+        return self.cfg.lk
+
+    @property
+    def s(self):
+        return self.cfg.s
+
+    @property
+    def P(self):
+        return self.cfg.P
+
+    @property
+    def n(self):
+        return self.cfg.n
+
+    def tinv(self, E: complex):
+        """t^-1(E) for interface sites and the two leads.
+
+        "diag" (nonrelativistic, config 0): E-independent diagonal blocks.
+        "herm"/"bdg": t^-1(E) = E*I + T0 with Hermitian T0, so that
+        t^-1 - G(k) = E - H(k) and Im E alone sets the absorption.
+        """
+        if self.cfg.tmodel == "diag":
+            return self.t0_int.copy(), self.t0_left.copy(), self.t0_right.copy()
+        eye = np.eye(self.lk)
+        return (self.t0_int + E * eye, self.t0_left + E * eye,
+                self.t0_right + E * eye)
+
+    def fingerprint(self) -> str:
+        h = hashlib.sha256()
+        for a in (self.R, self.G_intra, self.G_inter, self.t0_int,
+                  self.t0_left, self.t0_right):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()[:16]
+
+
+def make_system(cfg: Config) -> System:
+    if cfg.tmodel == "bdg":
+        h = cfg.lk // 2
+        R, gi, go = _gen_structure(cfg, h)
+        s = cfg.s
+
+        def bdg(g):
+            out = np.zeros(g.shape[:-2] + (s * cfg.lk, s * cfg.lk), np.complex128)
+            for a in range(s):
+                for b in range(s):
+                    blk = g[..., a * h:(a + 1) * h, b * h:(b + 1) * h]
+                    out[..., a * cfg.lk:a * cfg.lk + h, b * cfg.lk:b * cfg.lk + h] = blk
+                    out[..., a * cfg.lk + h:(a + 1) * cfg.lk,
+                        b * cfg.lk + h:(b + 1) * cfg.lk] = -blk.conj()
+            return out
+
+        gi, go = bdg(gi), bdg(go)
+    elif cfg.spin_diag:
+        norb = cfg.lk // 2
+        R, gi, go = _gen_structure(cfg, norb)
+        gi = _expand_spin(gi, cfg.s, norb)
+        go = _expand_spin(go, cfg.s, norb)
+    else:
+        R, gi, go = _gen_structure(cfg, cfg.lk)
+    rng_t = _rng(cfg, 1)
+    rng_l = _rng(cfg, 2)
+    t0_int = _gen_t0(cfg, rng_t, cfg.n_sites)
+    t0_left = _gen_t0(cfg, rng_l, cfg.s)
+    t0_right = _gen_t0(cfg, rng_l, cfg.s)
+    return System(cfg, R, gi, go, t0_int, t0_left, t0_right)
+
+
+def spin_channel(sys_: System, spin: int) -> System:
+    """Spin channel of a spin-diagonal (nonrelativistic) system: the
+    lk/2-wide problem the reference runs once per spin
+    (skkr_kpar_v6/nrsp-scf/skkr.f:258)."""
+    assert sys_.cfg.spin_diag
+    lk = sys_.lk
+    norb = lk // 2
+    sel = np.concatenate([a * lk + spin * norb + np.arange(norb)
+                          for a in range(sys_.s)])
+    osel = spin * norb + np.arange(norb)
+    cfg = dataclasses.replace(sys_.cfg, lk=norb, spin_diag=False,
+                              name=f"{sys_.cfg.name}-spin{spin + 1}")
+    return System(cfg, sys_.R,
+                  sys_.G_intra[:, sel][:, :, sel], sys_.G_inter[:, sel][:, :, sel],
+                  sys_.t0_int[:, osel][:, :, osel], sys_.t0_left[:, osel][:, :, osel],
+                  sys_.t0_right[:, osel][:, :, osel])

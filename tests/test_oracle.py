@@ -1,0 +1,209 @@
+"""The CPU oracle pinned against the reference's golden vectors, closed forms
+and dense linear algebra (no GPU)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from skkr_budapest_amd import systems as S
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _golden_systems():
+    c0 = S.make_system(S.CONFIGS[0])
+    cb = S.Config("golden-bdg18", 71, lk=18, s=2, P=4, r2max=4, tmodel="bdg",
+                  delta=0.05, energies=(0.35 + 0.06j,), mesh=6)
+    ch = S.Config("golden-herm16", 72, lk=16, s=4, P=4, r2max=8, tmodel="herm",
+                  energies=(0.4 + 0.08j,), mesh=4)
+    return {"cfg0_spin1": S.spin_channel(c0, 0), "cfg0_spin2": S.spin_channel(c0, 1),
+            "bdg18": S.make_system(cb), "herm16": S.make_system(ch)}
+
+
+GOLDEN = _golden_systems()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_oracle_matches_reference_golden(name):
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    sy = GOLDEN[name]
+    assert str(g["fingerprint"]) == sy.fingerprint(), "generator drifted"
+    r = oracle.tau_bz(sy, complex(g["E"]), g["k"], g["w"], nworkers=4, want_tau_k=True)
+    assert r["rc"] == 0
+    # decimation iteration counts identical to the reference's surfgfd
+    np.testing.assert_array_equal(r["iters"], g["iters"])
+    assert rel(r["tau_bz"], g["tau_bz"]) < 1e-12
+    assert np.abs(r["trk"] - g["trk"]).max() < 1e-12 * np.abs(g["trk"]).max()
+    for j, ik in enumerate(g["save_idx"]):
+        assert rel(r["tau_k"][ik], g["tau_save"][j]) < 1e-12
+
+
+def test_config0_equals_its_two_spin_channels():
+    """Config 0 (spin-diagonal G, 18x18 blocks) decouples into the two 9x9
+    channels the reference runs separately; the only difference is the
+    decimation stop rule measured over both spins (|alpha| < 1e-8)."""
+    cfg = S.CONFIGS[0]
+    sy = S.make_system(cfg)
+    k, w = S.uniform_kmesh(4)
+    full = oracle.tau_bz(sy, cfg.energies[0], k, w)["tau_bz"]
+    norb = cfg.lk // 2
+    for sp in range(2):
+        ch = oracle.tau_bz(S.spin_channel(sy, sp), cfg.energies[0], k, w)["tau_bz"]
+        blk = full[:, sp * norb:(sp + 1) * norb, sp * norb:(sp + 1) * norb]
+        assert rel(blk, ch) < 1e-7
+    off = full[:, :norb, norb:]
+    assert np.abs(off).max() < 1e-14
+
+
+def _dense_stack(sy, E, k):
+    gi = oracle.fourier(sy.R, sy.G_intra, k)
+    go = oracle.fourier(sy.R, sy.G_inter, k)
+    m00, m01, m10 = oracle.pl_blocks(gi, go)
+    ti, tl, tr = sy.tinv(E)
+    n, P, lk, s = sy.n, sy.P, sy.lk, sy.s
+
+    def bd(ts):
+        out = np.zeros((n, n), complex)
+        for a in range(s):
+            out[a * lk:(a + 1) * lk, a * lk:(a + 1) * lk] = ts[a]
+        return out
+
+    return m00, m01, m10, ti, tl, tr, bd
+
+
+def test_tau2d_equals_dense_inverse_of_finite_stack():
+    cfg = S.small_config(lk=3, s=2, P=5, r2max=2, energies=(0.3 + 0.1j,))
+    sy = S.make_system(cfg)
+    E, k = cfg.energies[0], np.array([0.4, -1.3])
+    m00, m01, m10, ti, tl, tr, bd = _dense_stack(sy, E, k)
+    Y, _, st1 = oracle.dugo(m00, m01, m10, tl, "L")
+    X, _, st2 = oracle.dugo(m00, m01, m10, tr, "R")
+    assert st1 == 0 and st2 == 0
+    tau, st = oracle.tau2d(m00, m01, m10, ti, cfg.P, X, Y, w=0.25)
+    assert st == 0
+    n, P, lk, s = sy.n, cfg.P, cfg.lk, cfg.s
+    A = np.zeros((P * n, P * n), complex)
+    for p in range(P):
+        A[p * n:(p + 1) * n, p * n:(p + 1) * n] = m00 + bd(ti[p * s:(p + 1) * s])
+        if p < P - 1:
+            A[p * n:(p + 1) * n, (p + 1) * n:(p + 2) * n] = m01
+            A[(p + 1) * n:(p + 2) * n, p * n:(p + 1) * n] = m10
+    A[:n, :n] -= Y
+    A[-n:, -n:] -= X
+    Ai = np.linalg.inv(A)
+    for p in range(P):
+        for a in range(s):
+            i0 = p * n + a * lk
+            np.testing.assert_allclose(tau[p * s + a], 0.25 * Ai[i0:i0 + lk, i0:i0 + lk],
+                                       atol=1e-13)
+
+
+def test_single_pl_tau_is_inverse_of_m00_minus_x_minus_y():
+    cfg = S.small_config(lk=4, s=1, P=1, r2max=1, energies=(0.2 + 0.3j,))
+    sy = S.make_system(cfg)
+    m00, m01, m10, ti, tl, tr, bd = _dense_stack(sy, cfg.energies[0], np.array([0.1, 0.2]))
+    Y, _, _ = oracle.dugo(m00, m01, m10, tl, "L")
+    X, _, _ = oracle.dugo(m00, m01, m10, tr, "R")
+    tau, _ = oracle.tau2d(m00, m01, m10, ti, 1, X, Y)
+    np.testing.assert_allclose(tau[0], np.linalg.inv(m00 + ti[0] - X - Y), atol=1e-13)
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_lead_self_energy_equals_long_finite_chain(side):
+    cfg = S.small_config(lk=2, s=2, P=1, r2max=2, energies=(0.1 + 0.25j,))
+    sy = S.make_system(cfg)
+    m00, m01, m10, ti, tl, tr, bd = _dense_stack(sy, cfg.energies[0], np.array([0.7, 0.2]))
+    t = tl if side == "L" else tr
+    sig, it, st = oracle.dugo(m00, m01, m10, t, side)
+    assert st == 0 and it > 2
+    n, L = sy.n, 300
+    mL = m00 + bd(t)
+    C = np.zeros((L * n, L * n), complex)
+    for p in range(L):
+        C[p * n:(p + 1) * n, p * n:(p + 1) * n] = mL
+        if p < L - 1:
+            C[p * n:(p + 1) * n, (p + 1) * n:(p + 2) * n] = m01
+            C[(p + 1) * n:(p + 2) * n, p * n:(p + 1) * n] = m10
+    Ci = np.linalg.inv(C)
+    if side == "L":   # left lead: its surface PL is the last one of the chain
+        ref = m10 @ Ci[-n:, -n:] @ m01
+    else:
+        ref = m01 @ Ci[:n, :n] @ m10
+    assert rel(sig, ref) < 1e-9
+
+
+def test_surfgfd_scalar_chain_closed_form():
+    """lk=1 chain with on-site a and hopping b: g = (a - sqrt(a^2-4b^2))/(2b^2),
+    the root with |b g| < 1."""
+    for a, b in [(0.3 + 0.05j, 0.5), (-1.1 + 0.2j, 0.4), (2.5 + 0.01j, 1.0)]:
+        g, it, st = oracle.surfgfd(np.array([[b]]), np.array([[a]]), np.array([[b]]),
+                                   errmax=1e-12)
+        assert st == 0
+        r = np.sqrt(a * a - 4 * b * b + 0j)
+        roots = [(a - r) / (2 * b * b), (a + r) / (2 * b * b)]
+        exact = min(roots, key=lambda z: abs(b * z))
+        assert abs(g[0, 0] - exact) < 1e-10 * abs(exact)
+
+
+def test_surfgfd_itermax_and_divergence_flags():
+    g, it, st = oracle.surfgfd(np.array([[0.5]]), np.array([[0.3 + 1e-4j]]),
+                               np.array([[0.5]]), itermax=1, errmax=1e-14)
+    assert st == 3 and it == 2
+
+
+def test_gjinv_against_numpy_with_pivoting():
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=(20, 20)) + 1j * rng.normal(size=(20, 20))
+    a[0, 0] = 1e-12  # forces a column exchange
+    inv, st = oracle.gjinv(a)
+    assert st == 0
+    np.testing.assert_allclose(inv @ a, np.eye(20), atol=1e-11)
+    z = np.zeros((3, 3), complex)
+    assert oracle.gjinv(z)[1] == 1
+
+
+def test_fourier_sum_closed_forms():
+    sy = S.make_system(S.small_config(lk=3, s=2, P=2, r2max=8))
+    R = sy.R
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        k = rng.uniform(-np.pi, np.pi, 2)
+        g = oracle.fourier(R, sy.G_intra, k)
+        bf = sum(sy.G_intra[r] * np.exp(-1j * (k @ R[r])) for r in range(len(R)))
+        np.testing.assert_allclose(g, bf, atol=1e-14)
+        np.testing.assert_allclose(g, g.conj().T, atol=1e-14)          # Hermitian
+        g2 = oracle.fourier(R, sy.G_intra, k + 2 * np.pi * np.array([1.0, -1.0]))
+        np.testing.assert_allclose(g, g2, atol=1e-12)                  # periodic
+        gs = oracle.fourier(R, 2.0 * sy.G_intra + 1j * sy.G_inter, k)
+        lin = 2.0 * g + 1j * oracle.fourier(R, sy.G_inter, k)
+        np.testing.assert_allclose(gs, lin, atol=1e-13)                 # linear
+    np.testing.assert_allclose(oracle.fourier(R, sy.G_inter, np.zeros(2)),
+                               sy.G_inter.sum(0), atol=1e-14)            # k = 0
+
+
+def test_pl_blocks_signs_and_hermitian_pair():
+    sy = S.make_system(S.small_config(lk=2, s=3, P=2, r2max=2))
+    k = np.array([0.3, 0.9])
+    gi = oracle.fourier(sy.R, sy.G_intra, k)
+    go = oracle.fourier(sy.R, sy.G_inter, k)
+    m00, m01, m10 = oracle.pl_blocks(gi, go)
+    np.testing.assert_array_equal(m00, -gi)
+    np.testing.assert_array_equal(m01, -go)
+    np.testing.assert_array_equal(m10, m01.conj().T)
+
+
+def test_round_robin_workers_sum_like_allreduce():
+    cfg = S.small_config(lk=2, s=2, P=3, r2max=2, energies=(0.2 + 0.2j,))
+    sy = S.make_system(cfg)
+    k, w = S.uniform_kmesh(4)
+    a = oracle.tau_bz(sy, cfg.energies[0], k, w, nworkers=1)
+    b = oracle.tau_bz(sy, cfg.energies[0], k, w, nworkers=3)
+    assert rel(b["tau_bz"], a["tau_bz"]) < 1e-14
+    np.testing.assert_array_equal(a["iters"], b["iters"])
+    # constant-tau invariance: BZ weights sum to one
+    assert abs(w.sum() - 1.0) < 1e-15
