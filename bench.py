@@ -1,0 +1,126 @@
+"""Benchmark of the per-(E,k) layer kernel on one MI355X.
+
+A step is one pass of the hot path over one energy of the workload: the
+k-loop body for every k-point of the mesh (lattice Fourier sum, decimation of
+both leads, Godfrin sweeps, BZ accumulation), inputs resident in HBM.
+
+Default workload: the relativistic layer system with lk=32, 4 sites per
+principal layer (n=128), 20 interface PLs, 25 in-plane vectors, one energy
+E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points).
+
+Prints one JSON line (rank 0).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("BZ-integrated tau k-points/s per energy: mixed+refine vs FP64, "
+          "at 1e-8 rel err")
+PEAK_FP64_TFLOPS = 78.6
+
+
+def algorithmic_flops_per_k(n, P, it_l, it_r):
+    """(56P-48) n^3 for the sweeps + (56 I + 24) n^3 per lead."""
+    return ((56 * P - 48) + (56 * it_l + 24) + (56 * it_r + 24)) * float(n) ** 3
+
+
+def cpu_baseline(sy, E, k, w):
+    import oracle
+    nworkers = min(16, os.cpu_count() or 1)
+    nk = 4 * nworkers
+    idx = np.linspace(0, len(w) - 1, nk).astype(int)
+    t0 = time.perf_counter()
+    oracle.tau_bz(sy, E, k[idx], w[idx], nworkers=nworkers)
+    dt = time.perf_counter() - t0
+    return {"value": nk / dt, "unit": "k-points/s", "cores": nworkers, "kind": "port",
+            "sample": f"{nk} k-points of the same energy and mesh, FP64 C oracle, "
+                      f"{nworkers} threads, round-robin k split, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the k-points a multi-GPU run would shard are all taken by one device.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from skkr_budapest_amd import gpu
+    from skkr_budapest_amd import systems as S
+
+    cfg = S.CONFIGS[args.config]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    nk = len(w)
+    print(f"bench: config {args.config} n={cfg.n} P={cfg.P} nk={nk} E={E}", file=sys.stderr,
+          flush=True)
+
+    ctx = gpu.Context.for_system(sy)
+    ctx.set_tinv(*sy.tinv(E))
+    for i in range(args.warmup):
+        r = ctx.tau_bz(E, k, w)
+        print(f"bench: warmup {i} {r['stats']['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    times, stats = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        r = ctx.tau_bz(E, k, w, timing_detail=True)
+        times.append(time.perf_counter() - t0)
+        stats.append(r["stats"])
+        print(f"bench: step {i} {times[-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr,
+              flush=True)
+    ms = 1e3 * float(np.mean(times))
+    st = stats[-1]
+    it_l, it_r = st["mean_iter_l"], st["mean_iter_r"]
+    dom_launch_ms = sum(s["dom_ms"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
+    dom_flops = sum(s["dom_flops"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
+    achieved = dom_flops / (dom_launch_ms * 1e-3) / 1e12 if dom_launch_ms > 0 else 0.0
+    value = nk / (ms * 1e-3)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "k-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded G_r(R), t^-1)",
+        "config": {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} "
+                               f"n={cfg.n} N_R={cfg.n_R} E={E} {cfg.mesh}x{cfg.mesh} k",
+                   "precision": "fp64", "nk": nk,
+                   "mean_iter_l": it_l, "mean_iter_r": it_r,
+                   "algorithmic_tflops": nk * algorithmic_flops_per_k(cfg.n, cfg.P, it_l, it_r)
+                   / (ms * 1e-3) / 1e12,
+                   "phase_ms": {p: st[f"ms_{p}"] for p in
+                                ("fourier", "decimation", "sweeps", "accumulate")}},
+        "roofline": {"bound": "mfma", "kernel": st["dom_name"], "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "avg_launch_us": dom_launch_ms * 1e3},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sy, E, k, w)
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * skkr_gpu.h -- C-ABI of the MI355X (gfx950) per-(E,k) layer kernel.
+ *
+ * Drop-in for the k-loop body of the reference's cpacoord
+ * (skkr_kpar_v6/nrsp-scf/cpacoord.f:147-317,
+ *  skkr_kpar_v6/rsp-scf/cpacoord_ksave.f:188-400,
+ *  skkr_dbogo_bsf/rsp-scf/cpacoord_ksave.f:289-867): per k-point the layer
+ * structure constants G(k), the two lead self-energies by decimation (dugo ->
+ * surfgfd), the Godfrin block-tridiagonal sweeps (tau2d) and the BZ-weighted
+ * sum of the site-diagonal blocks of tau.
+ *
+ * Conventions (those of the reference):
+ *  - complex arrays are complex*16: interleaved (re, im) doubles, Fortran
+ *    column-major per matrix, no padding (leading dimension = matrix width);
+ *  - site-major ordering inside a principal layer (PL), row index
+ *    kk = site*lk + orbital (skkr_kpar_v6/main/packer.f:47-95);
+ *  - the caller owns every host array; the context owns device memory;
+ *  - functions return 0 (SKKR_OK) or an SKKR_ERR_* code; where the reference
+ *    stops (decimation non-convergence, cpacoord.f:233-238, surfgf.f:100-103)
+ *    the per-k status array names the failing k and the host decides.
+ *
+ * Every entry point is callable from C, C++ and Fortran (ISO_C_BINDING,
+ * module skkr_gpu in skkr_budapest_amd/fortran/skkr_gpu_mod.f90).
+ */
+#ifndef SKKR_GPU_H
+#define SKKR_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SKKR_OK = 0,
+  SKKR_ERR_ARG = 1,          /* bad argument or size */
+  SKKR_ERR_HIP = 2,          /* HIP runtime error (no device, launch failure) */
+  SKKR_ERR_NOMEM = 3,        /* device allocation failed */
+  SKKR_ERR_SINGULAR = 4,     /* zero pivot in an inversion */
+  SKKR_ERR_SGF_DIVERGED = 5, /* decimation |alpha|_F > 1e10 (surfgf.f:89-94) */
+  SKKR_ERR_SGF_ITERMAX = 6,  /* decimation iter > itermax (surfgf.f:100-103) */
+  SKKR_ERR_UNSUPPORTED = 7   /* size/mode not built */
+};
+
+/* precision modes */
+enum {
+  SKKR_FP64 = 0,        /* every product and inversion in FP64 */
+  SKKR_MIXED_FP32 = 1,  /* FP32 MFMA products/inversions + FP64 refinement */
+  SKKR_MIXED_BF16 = 2   /* split-BF16 MFMA products + FP64 refinement */
+};
+
+typedef struct skkr_ctx skkr_ctx;
+
+typedef struct {
+  int lk;              /* site block width (kmymax / lmmax of the reference) */
+  int sites_per_pl;    /* ninprc (equal for every PL and both leads) */
+  int n_pl;            /* nprc: interface principal layers */
+  double errmax;       /* decimation stop |alpha|_F < errmax (/iterpar/) */
+  int itermax_l;       /* itermaxl */
+  int itermax_r;       /* itermaxr */
+  int precision;       /* SKKR_FP64 | SKKR_MIXED_FP32 | SKKR_MIXED_BF16 */
+  double fp64_dec_below_imE; /* mixed modes: FP64 decimation when Im E < this */
+  int device;          /* HIP device ordinal */
+  int batch;           /* k-points per launch group; 0 = automatic */
+} skkr_config;
+
+typedef struct {
+  double ms_total;        /* wall time of the call, host clock */
+  double ms_fourier;      /* device time per phase, HIP events on ctx stream */
+  double ms_decimation;
+  double ms_sweeps;
+  double ms_accumulate;
+  double ms_refine;
+  double mean_iter_l, mean_iter_r;
+  int max_iter;
+  int n_batches;
+  long long n_launches;
+  /* dominant kernel: launches and summed event time (HIP events bracketing
+   * each launch on the ctx stream; filled only when timing_detail != 0) */
+  int timing_detail;
+  long long dom_launches;
+  double dom_ms;
+  double dom_flops;       /* algorithmic flops of those launches */
+  char dom_name[32];
+} skkr_stats;
+
+/* Context: one per host thread / rank; calls on a context are serialised on
+ * its HIP stream. */
+int skkr_create(skkr_ctx **ctx, const skkr_config *cfg);
+void skkr_destroy(skkr_ctx *ctx);
+const char *skkr_error_string(int code);
+
+/* Real-space structure constants of the bulk-like stack (replaces the G(k)
+ * producer gstore -> kamstr/directg -> scrstr -> getg2d,
+ * skkr_kpar_v6/main/gstore.f:6-283).  R: 2*n_R doubles (x,y per vector);
+ * GR_intra, GR_inter: n_R matrices n x n (n = lk*sites_per_pl) of the
+ * coupling inside a PL and from PL p to PL p+1; G(k) = sum_R G_r(R) e^{-ik.R},
+ * the coupling from PL p+1 back to PL p is G_inter(-R)^dagger. */
+int skkr_set_structure(skkr_ctx *ctx, int n_R, const double *R,
+                       const double *GR_intra, const double *GR_inter);
+
+/* Inverse single-site t-matrices at the current energy (tminv, tminvl,
+ * tminvr of cpacoord.f:31-35): lk x lk per site, n_pl*sites_per_pl interface
+ * sites then sites_per_pl sites for each lead. */
+int skkr_set_tinv(skkr_ctx *ctx, const double *tinv_int, const double *tinv_l,
+                  const double *tinv_r);
+
+/* The k-loop body for nk k-points at energy E (cpacoord.f:147-317):
+ * tau_bz (lk x lk x n_pl*sites_per_pl) += sum_k w_k tau_site(k).
+ * Optional outputs (NULL to skip): tr_tau_k (nk x sites, complex) traces of the
+ * unweighted site-diagonal tau per k (taukdiag of cpacoord.f:300-309 summed
+ * over l), iters_k (2*nk: left, right decimation iterations), status_k (nk). */
+int skkr_tau_bz(skkr_ctx *ctx, double e_re, double e_im, int nk,
+                const double *k, const double *w, double *tau_bz,
+                double *tr_tau_k, int *iters_k, int *status_k,
+                skkr_stats *stats);
+
+/* ---- per-routine batched entry points (host arrays in and out) ---------- */
+
+/* G(k) -> PL blocks for nk k-points (the gstore+packer step,
+ * skkr_kpar_v6/main/packer.f:6-100): m00 = -G_intra(k), m01 = -G_inter(k),
+ * m10 = m01^dagger, each nk x (n x n). */
+int skkr_pl_blocks(skkr_ctx *ctx, int nk, const double *k, double *m00,
+                   double *m01, double *m10);
+
+/* In-place inverse of nb matrices n x n (gjinv, skkr_kpar_v6/main/gjinv.f:2);
+ * n <= 128.  status (nb, optional) gets SKKR_ERR_SINGULAR per matrix. */
+int skkr_gjinv_batch(skkr_ctx *ctx, int n, int nb, double *a, int *status);
+
+/* c = a*b for nb pairs of n x n matrices (doubmt2, matops.f:204-228). */
+int skkr_zgemm_batch(skkr_ctx *ctx, int n, int nb, const double *a,
+                     const double *b, double *c);
+
+/* Decimation for nb independent semi-infinite chains (surfgfd,
+ * skkr_kpar_v6/main/surfgf.f:2): x = surface Green's function, iters and
+ * status per chain. */
+int skkr_surfgfd_batch(skkr_ctx *ctx, int nb, const double *ml,
+                       const double *m0, const double *mr, double *x,
+                       int *iters, int *status);
+
+/* Lead self-energy for nb k-points (dugo bulk branch,
+ * skkr_kpar_v6/main/dugo.f:9): side 'L' -> Y_1 = m10 g m01, 'R' -> X_P =
+ * m01 g m10, with g the surface Green's function of m00 + tinv_lead. */
+int skkr_dugo_batch(skkr_ctx *ctx, char side, int nb, const double *m00,
+                    const double *m01, const double *m10, double *xy,
+                    int *iters, int *status);
+
+/* Godfrin sweeps for nb k-points (tau2d, skkr_kpar_v6/main/tau2d.f:2) with
+ * the interface t^-1 set by skkr_set_tinv: tau (nb x sites x lk x lk) =
+ * w * site-diagonal blocks of the PL-diagonal tau. */
+int skkr_tau2d_batch(skkr_ctx *ctx, int nb, const double *m00,
+                     const double *m01, const double *m10, const double *x,
+                     const double *y, double w, double *tau, int *status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SKKR_GPU_H */

@@ -1,0 +1,450 @@
+// FP64 device kernels of the per-(E,k) layer kernel (gfx950).
+//
+// Matrices are complex*16 column-major, one n x n block per k-point (or per
+// decimation problem), addressed through per-batch pointers or strides.
+#pragma once
+#include <hip/hip_runtime.h>
+
+typedef __attribute__((ext_vector_type(4))) double dx4;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+  return make_double2(a.x - b.x, a.y - b.y);
+}
+// 1/q with the scaling of Smith's method (no overflow for large |q|)
+__device__ __forceinline__ double2 crecip(double2 q) {
+  if (fabs(q.x) >= fabs(q.y)) {
+    double r = q.y / q.x, d = q.x + q.y * r;
+    return make_double2(1.0 / d, -r / d);
+  } else {
+    double r = q.x / q.y, d = q.x * r + q.y;
+    return make_double2(r / d, -1.0 / d);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lattice Fourier sum: out[b][e] = scale * sum_r GR[r][e] * exp(i*sgn*(k_b.R_r))
+// One thread per matrix element e; GR[.][e] held in registers; the phases of
+// KB k-points staged in LDS; consecutive threads write consecutive elements.
+// ---------------------------------------------------------------------------
+template <int MAXR, int KB>
+__global__ __launch_bounds__(256) void fourier_kernel(
+    int nn, int nR, const double2* __restrict__ GR, const double* __restrict__ R,
+    const double* __restrict__ kpts, int nk, double sgn, double scale,
+    double2* __restrict__ out, long long out_stride) {
+  __shared__ double2 ph[KB * MAXR];
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int k0 = blockIdx.y * KB;
+  for (int t = threadIdx.x; t < KB * nR; t += 256) {
+    int kb = t / nR, r = t % nR;
+    int kk = k0 + kb;
+    double phi = 0.0;
+    if (kk < nk) phi = kpts[2 * kk] * R[2 * r] + kpts[2 * kk + 1] * R[2 * r + 1];
+    double sn, cs;
+    sincos(phi, &sn, &cs);
+    ph[kb * MAXR + r] = make_double2(cs, sgn * sn);
+  }
+  __syncthreads();
+  if (e >= nn) return;
+  double2 g[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r)
+    g[r] = (r < nR) ? GR[(long long)r * nn + e] : make_double2(0.0, 0.0);
+  const int kend = min(KB, nk - k0);
+  for (int kb = 0; kb < kend; ++kb) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) {
+      if (r < nR) {
+        double2 p = ph[kb * MAXR + r];
+        acc.x += g[r].x * p.x - g[r].y * p.y;
+        acc.y += g[r].x * p.y + g[r].y * p.x;
+      }
+    }
+    out[(long long)(k0 + kb) * out_stride + e] = make_double2(scale * acc.x, scale * acc.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched complex GEMM on FP64 MFMA:
+//   C[b] = alpha * A[b](m x k) * B[b](k x n) + beta * C[b]
+// Work-group tile TM x TN, KC-deep LDS stages, each wave a 32x32 sub-tile of
+// 2x2 v_mfma_f64_16x16x4 blocks.  The MFMA computes the transposed block
+// (B^T A^T) so that lanes 0..15 own 16 consecutive rows of C and the stores
+// of a column are contiguous.
+// Complex product with two accumulators: re += ar*br - ai*bi, im += ar*bi + ai*br.
+// ---------------------------------------------------------------------------
+template <int TM, int TN, int KC>
+__global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
+    int m, int n, int k, const double2* const* __restrict__ Ap, int lda,
+    const double2* const* __restrict__ Bp, int ldb, double2* const* __restrict__ Cp,
+    int ldc, double2 alpha, double2 beta, const int* __restrict__ active) {
+  constexpr int NW = (TM / 32) * (TN / 32);
+  constexpr int NT = NW * 64;
+  __shared__ double2 As[KC][TM];
+  __shared__ double2 Bs[TN][KC + 1];
+  const int b = blockIdx.y;
+  if (active && !active[b]) return;
+  const int tiles_m = (m + TM - 1) / TM;
+  const int row0 = (blockIdx.x % tiles_m) * TM;
+  const int col0 = (blockIdx.x / tiles_m) * TN;
+  const double2* __restrict__ A = Ap[b];
+  const double2* __restrict__ B = Bp[b];
+  double2* __restrict__ C = Cp[b];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w % (TM / 32), wn = w / (TM / 32);
+  dx4 acc_re[2][2], acc_im[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc_re[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
+      acc_im[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
+    }
+  const int l16 = lane & 15, l4 = lane >> 4;
+  for (int k0 = 0; k0 < k; k0 += KC) {
+    for (int idx = tid; idx < TM * KC; idx += NT) {
+      int r = idx % TM, kk = idx / TM;
+      int gr = row0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < m && gk < k) ? A[gr + (long long)gk * lda] : make_double2(0.0, 0.0);
+    }
+    for (int idx = tid; idx < TN * KC; idx += NT) {
+      int kk = idx % KC, c = idx / KC;
+      int gk = k0 + kk, gc = col0 + c;
+      Bs[c][kk] = (gk < k && gc < n) ? B[gk + (long long)gc * ldb] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 4) {
+      double2 af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = As[ks + l4][wm * 32 + i * 16 + l16];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[wn * 32 + j * 16 + l16][ks + l4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          // transposed block: MFMA A-operand = B^T (lane -> col of C),
+          // B-operand = A^T (lane -> row of C)
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].x, acc_re[i][j], 0, 0, 0);
+          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bf[j].y, af[i].y, acc_re[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].y, acc_im[i][j], 0, 0, 0);
+          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].y, af[i].x, acc_im[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // D^T layout: lane l, register q -> C row (l % 16), C column 4*(l/16) + q
+  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int gr = row0 + wm * 32 + i * 16 + l16;
+        int gc = col0 + wn * 32 + j * 16 + 4 * l4 + q;
+        if (gr < m && gc < n) {
+          double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
+          double2 r = cmul(alpha, v);
+          long long o = gr + (long long)gc * ldc;
+          if (use_beta) r = cadd(r, cmul(beta, C[o]));
+          C[o] = r;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Batched in-place Gauss-Jordan inversion, register resident, n <= TR*RPT and
+// n <= TC*CPT.  Thread (tr, tc) owns rows tr + TR*i and columns tc + TC*j.
+// Partial pivoting over unused rows of the current column with implicit row
+// exchange: with pivot row pi(i) for column i the stored S gives
+// A^-1[pinv(rho)][pi(gamma)] = S[rho][gamma].
+//
+// The matrix is assembled on load:
+//   a = base + blockdiag(tinv) - sub1 - sub2      (any term optional)
+// Epilogue: mode 0 writes the full inverse to out; mode 1 writes only the
+// site-diagonal lk x lk blocks to tau (lk*lk per site, sites consecutive) and
+// the traces of those blocks to tr.
+// ---------------------------------------------------------------------------
+struct InvArgs {
+  int n, lk;
+  const double2* const* base; // per batch (may repeat)
+  const double2* tinv;        // site blocks lk*lk, shared by the batch (or null)
+  const double2* const* sub1; // or null
+  const double2* const* sub2; // or null
+  double2* const* out;        // mode 0
+  double2* const* tau;        // mode 1
+  double2* const* tr;         // mode 1 (n/lk per batch)
+  const int* active;          // or null
+  int* status;                // or null; |= 1 on zero pivot
+  int mode;
+};
+
+template <int TR, int TC, int RPT, int CPT>
+__global__ __launch_bounds__(TR * TC) void gjinv_f64_kernel(InvArgs args) {
+  constexpr int NT = TR * TC;
+  constexpr int NMAX = (TR * RPT < TC * CPT) ? TR * RPT : TC * CPT;
+  __shared__ double2 colbuf[2][NMAX];
+  __shared__ double2 rowbuf[2][NMAX];
+  __shared__ int rowused[NMAX];
+  __shared__ int perm[NMAX];
+  __shared__ int pinv[NMAX];
+  __shared__ double2 diag[NMAX];
+  const int b = blockIdx.x;
+  if (args.active && !args.active[b]) return;
+  const int n = args.n;
+  const int tid = threadIdx.x;
+  const int tr = tid % TR, tc = tid / TR;
+  const int lane = tid & 63;
+  double2 a[RPT][CPT];
+  {
+    const double2* __restrict__ base = args.base[b];
+    const double2* __restrict__ s1 = args.sub1 ? args.sub1[b] : nullptr;
+    const double2* __restrict__ s2 = args.sub2 ? args.sub2[b] : nullptr;
+    const int lk = args.lk;
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        int r = tr + TR * i, c = tc + TC * j;
+        double2 v = make_double2(0.0, 0.0);
+        if (r < n && c < n) {
+          long long o = r + (long long)c * n;
+          v = base[o];
+          if (args.tinv && (r / lk) == (c / lk)) {
+            int st = r / lk;
+            v = cadd(args.tinv[(long long)st * lk * lk + (r % lk) + (c % lk) * lk], v);
+          }
+          if (s1) v = csub(v, s1[o]);
+          if (s2) v = csub(v, s2[o]);
+        }
+        a[i][j] = v;
+      }
+  }
+  for (int t = tid; t < NMAX; t += NT) rowused[t] = (t < n) ? 0 : 1;
+  int singular = 0;
+  // column 0 into colbuf[0]
+#pragma unroll
+  for (int j = 0; j < CPT; ++j)
+    if (tc + TC * j == 0)
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+        if (tr + TR * i < n) colbuf[0][tr + TR * i] = a[i][j];
+  __syncthreads();
+  for (int it = 0; it < n; ++it) {
+    const int cur = it & 1;
+    // pivot: every wave computes argmax over unused rows (same result)
+    double best = -1.0;
+    int bi = NMAX;
+    for (int r = lane; r < n; r += 64) {
+      if (!rowused[r]) {
+        double2 v = colbuf[cur][r];
+        double mg = v.x * v.x + v.y * v.y;
+        if (mg > best) { best = mg; bi = r; }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double ob = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    int p = bi;
+    if (p >= n) {  // only when every candidate is NaN: take the first unused row
+      p = 0;
+      while (p < n - 1 && rowused[p]) ++p;
+      singular = 1;
+    }
+    const double2 piv = colbuf[cur][p];
+    if (piv.x == 0.0 && piv.y == 0.0) singular = 1;
+    const double2 q = crecip(piv);
+    // scale pivot row, publish it
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      if (tr + TR * i == p) {
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+          int c = tc + TC * j;
+          if (c < n) {
+            double2 v = (c == it) ? q : cmul(a[i][j], q);
+            a[i][j] = v;
+            rowbuf[cur][c] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // eliminate every other row; owners of column it+1 publish it
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = tr + TR * i;
+      if (r < n && r != p) {
+        const double2 f = colbuf[cur][r];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+          const int c = tc + TC * j;
+          if (c < n) {
+            const double2 rb = rowbuf[cur][c];
+            double2 v = (c == it) ? make_double2(0.0, 0.0) : a[i][j];
+            v.x -= f.x * rb.x - f.y * rb.y;
+            v.y -= f.x * rb.y + f.y * rb.x;
+            a[i][j] = v;
+          }
+        }
+      }
+    }
+    if (it + 1 < n) {
+#pragma unroll
+      for (int j = 0; j < CPT; ++j)
+        if (tc + TC * j == it + 1)
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (tr + TR * i < n) colbuf[cur ^ 1][tr + TR * i] = a[i][j];
+    }
+    if (tid == 0) {
+      rowused[p] = 1;
+      perm[it] = p;
+      pinv[p] = it;
+    }
+    __syncthreads();
+  }
+  if (singular && args.status && tid == 0) atomicOr(&args.status[b], 1);
+  if (args.mode == 0) {
+    double2* __restrict__ out = args.out[b];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        int rho = tr + TR * i, gam = tc + TC * j;
+        if (rho < n && gam < n) out[pinv[rho] + (long long)perm[gam] * n] = a[i][j];
+      }
+  } else {
+    const int lk = args.lk;
+    double2* __restrict__ tau = args.tau[b];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        int rho = tr + TR * i, gam = tc + TC * j;
+        if (rho < n && gam < n) {
+          int rr = pinv[rho], cc = perm[gam];
+          if (rr / lk == cc / lk) {
+            int st = rr / lk;
+            tau[(long long)st * lk * lk + (rr % lk) + (cc % lk) * lk] = a[i][j];
+            if (rr == cc) diag[rr] = a[i][j];
+          }
+        }
+      }
+    __syncthreads();
+    const int ns = n / lk;
+    for (int st = tid; st < ns; st += NT) {
+      double2 s = make_double2(0.0, 0.0);
+      for (int l = 0; l < lk; ++l) s = cadd(s, diag[st * lk + l]);
+      args.tr[b][st] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Decimation helpers (surfgf.f): one work-group per chain.
+// ---------------------------------------------------------------------------
+// init: AB = [-ml; -mr] (2n x n, ld 2n), eps = x = m0 + blockdiag(tinv)
+__global__ __launch_bounds__(256) void dec_init_kernel(
+    int n, int lk, const double2* const* ml, const double2* const* mr,
+    const double2* const* m0, const double2* const* tinv, double2* const* AB,
+    double2* const* eps, double2* const* x) {
+  const int b = blockIdx.x;
+  const double2* __restrict__ L = ml[b];
+  const double2* __restrict__ Rm = mr[b];
+  const double2* __restrict__ M0 = m0[b];
+  const double2* __restrict__ T = tinv ? tinv[b] : nullptr;
+  for (int e = threadIdx.x; e < n * n; e += 256) {
+    int r = e % n, c = e / n;
+    double2 v = M0[e];
+    if (T && r / lk == c / lk)
+      v = cadd(T[(long long)(r / lk) * lk * lk + (r % lk) + (c % lk) * lk], v);
+    eps[b][e] = v;
+    x[b][e] = v;
+    AB[b][r + (long long)c * 2 * n] = make_double2(-L[e].x, -L[e].y);
+    AB[b][n + r + (long long)c * 2 * n] = make_double2(-Rm[e].x, -Rm[e].y);
+  }
+}
+
+// After C1 = [ay;by]*alpha, C2 = [ay;by]*beta (2n x n, ld 2n):
+//   alpha' = ay*alpha (C1 top), beta' = by*beta (C2 bottom),
+//   eps <- eps - ay*beta - by*alpha, x <- x - ay*beta, err = |alpha'|_F.
+// Convergence bookkeeping of surfgf.f:76-103 per chain (chains b < nhalf are
+// left leads with itermax_l, the others right leads with itermax_r).
+__global__ __launch_bounds__(256) void dec_update_kernel(
+    int n, int iter, int itermax_l, int itermax_r, int nhalf, double errmax,
+    const double2* const* C1,
+    const double2* const* C2, double2* const* AB, double2* const* eps,
+    double2* const* x, int* active, int* iters, int* status, int* n_active) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  if (!active[b]) return;
+  double sum = 0.0;
+  const double2* __restrict__ c1 = C1[b];
+  const double2* __restrict__ c2 = C2[b];
+  for (int e = threadIdx.x; e < n * n; e += 256) {
+    int r = e % n, c = e / n;
+    long long top = r + (long long)c * 2 * n, bot = top + n;
+    double2 an = c1[top], bn = c2[bot], y3 = c2[top], y4 = c1[bot];
+    AB[b][top] = an;
+    AB[b][bot] = bn;
+    eps[b][e] = csub(csub(eps[b][e], y3), y4);
+    x[b][e] = csub(x[b][e], y3);
+    sum += an.x * an.x + an.y * an.y;
+  }
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double err = sqrt(red[0]);
+    const int itermax = (b < nhalf) ? itermax_l : itermax_r;
+    if (err < errmax || iter > itermax) {
+      active[b] = 0;
+      iters[b] = iter;
+      if (iter > itermax) status[b] |= 4;
+    } else if (err > 1.0e10) {
+      active[b] = 0;
+      iters[b] = iter;
+      status[b] |= 2;
+    } else {
+      atomicAdd(n_active, 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BZ accumulation: acc[e] += sum_b w[b] * tau[b][e], b in order (deterministic)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bz_accumulate_kernel(
+    int ne, int nb, const double2* __restrict__ tau, long long stride,
+    const double* __restrict__ w, double2* __restrict__ acc) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  double2 s = acc[e];
+  for (int b = 0; b < nb; ++b) {
+    double2 t = tau[(long long)b * stride + e];
+    s.x += w[b] * t.x;
+    s.y += w[b] * t.y;
+  }
+  acc[e] = s;
+}
+
+__global__ __launch_bounds__(256) void scale_copy_kernel(
+    long long ne, const double2* __restrict__ src, double scale, double2* __restrict__ dst) {
+  long long e = blockIdx.x * 256ll + threadIdx.x;
+  if (e < ne) dst[e] = make_double2(scale * src[e].x, scale * src[e].y);
+}

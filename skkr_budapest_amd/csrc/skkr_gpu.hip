@@ -1,0 +1,811 @@
+// skkr_gpu.hip -- host orchestration and C-ABI of the per-(E,k) layer kernel.
+//
+// One context owns a HIP stream and the device buffers of a k batch.  Per
+// batch of B k-points:
+//   1. lattice Fourier sum -> M00' = -G_intra(k), M01 = -G_inter(k),
+//      M10 = M01^dagger                                   (gstore+packer)
+//   2. decimation of both leads as 2B independent chains   (dugo/surfgfd)
+//   3. lead self-energies Y_1 = M10 g_L M01, X_P = M01 g_R M10
+//   4. Godfrin sweeps; the PL-diagonal inversion is fused into the X sweep
+//      and writes only the site-diagonal blocks              (tau2d)
+//   5. BZ accumulation in FP64, k summed in order            (cpacoord BZ sum)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/skkr_gpu.h"
+#include "kernels_f64.h"
+
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) {                                                \
+      fprintf(stderr, "skkr_gpu: %s failed: %s (%s:%d)\n", #x,             \
+              hipGetErrorString(e_), __FILE__, __LINE__);                  \
+      return SKKR_ERR_HIP;                                                 \
+    }                                                                      \
+  } while (0)
+
+#define CHK(x)                  \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != SKKR_OK) return r_; \
+  } while (0)
+
+typedef double2 z;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct PtrArr {  // device array of matrix pointers
+  z** d = nullptr;
+  int n = 0;
+};
+
+}  // namespace
+
+struct skkr_ctx {
+  skkr_config cfg;
+  int n = 0, nn = 0, ns = 0, nR = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // structure and t^-1
+  double* dR = nullptr;
+  z* dGRi = nullptr;  // intra
+  z* dGRo = nullptr;  // inter (p -> p+1)
+  z* dGRb = nullptr;  // inter^dagger per R (p+1 -> p), phase e^{+ik.R}
+  z* dTint = nullptr;
+  z* dTl = nullptr;
+  z* dTr = nullptr;
+  bool have_structure = false, have_tinv = false;
+  // batch buffers
+  int B = 0;
+  double* dk = nullptr;
+  double* dw = nullptr;
+  z *M00 = nullptr, *M01 = nullptr, *M10 = nullptr;
+  z *eps = nullptr, *xs = nullptr, *AB = nullptr, *Yinv = nullptr, *AYBY = nullptr,
+    *C1 = nullptr, *C2 = nullptr;
+  int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
+      *nactive = nullptr, *status_sw = nullptr;
+  z *Ys = nullptr, *W = nullptr, *Zb = nullptr, *Xc = nullptr, *tauk = nullptr,
+    *trk = nullptr;
+  z* taubz = nullptr;
+  // pointer arrays
+  PtrArr p_ml, p_mr, p_m0, p_tl, p_AB, p_eps, p_x, p_Yinv, p_AYBY, p_C1, p_C2,
+      p_alpha, p_beta, p_sigma;
+  PtrArr p_M00, p_M01, p_M10, p_W, p_Z, p_X;
+  std::vector<PtrArr> p_Ys, p_tau, p_tr;
+  // stats
+  long long launches = 0;
+  int timing_detail = 0;
+  std::vector<hipEvent_t> evpool;  // pairs bracketing each dominant-kernel launch
+  std::vector<double> evflops;
+  size_t evused = 0;
+  double dom_ms = 0.0, dom_flops = 0.0;
+  long long dom_launches = 0;
+};
+
+// ------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------
+static int dalloc(skkr_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) return SKKR_ERR_NOMEM;
+  c->allocs.push_back(*p);
+  return SKKR_OK;
+}
+template <class T>
+static int dalloc_t(skkr_ctx* c, T** p, size_t count) {
+  return dalloc(c, (void**)p, count * sizeof(T));
+}
+
+static int make_ptrs(skkr_ctx* c, PtrArr* pa, const std::vector<z*>& v) {
+  CHK(dalloc_t(c, &pa->d, v.size()));
+  pa->n = (int)v.size();
+  HIPCHK(hipMemcpy(pa->d, v.data(), v.size() * sizeof(z*), hipMemcpyHostToDevice));
+  return SKKR_OK;
+}
+
+static int free_batch(skkr_ctx* c) {
+  // buffers live until destroy; a new batch size reallocates everything
+  return SKKR_OK;
+}
+
+static int ensure_batch(skkr_ctx* c, int B) {
+  if (c->B >= B) return SKKR_OK;
+  free_batch(c);
+  const int n = c->n;
+  const size_t nn = c->nn, P = c->cfg.n_pl, lk = c->cfg.lk, ns = c->ns;
+  const size_t B2 = 2 * (size_t)B;
+  CHK(dalloc_t(c, &c->dk, 2 * (size_t)B));
+  CHK(dalloc_t(c, &c->dw, (size_t)B));
+  CHK(dalloc_t(c, &c->M00, B * nn));
+  CHK(dalloc_t(c, &c->M01, B * nn));
+  CHK(dalloc_t(c, &c->M10, B * nn));
+  CHK(dalloc_t(c, &c->eps, B2 * nn));
+  CHK(dalloc_t(c, &c->xs, B2 * nn));
+  CHK(dalloc_t(c, &c->AB, B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->Yinv, B2 * nn));
+  CHK(dalloc_t(c, &c->AYBY, B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->C1, B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->C2, B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->active, B2));
+  CHK(dalloc_t(c, &c->iters, B2));
+  CHK(dalloc_t(c, &c->status, B2));
+  CHK(dalloc_t(c, &c->itermax, B2));
+  CHK(dalloc_t(c, &c->nactive, 1));
+  CHK(dalloc_t(c, &c->status_sw, (size_t)B));
+  CHK(dalloc_t(c, &c->Ys, P * B * nn));
+  CHK(dalloc_t(c, &c->W, B * nn));
+  CHK(dalloc_t(c, &c->Zb, B * nn));
+  CHK(dalloc_t(c, &c->Xc, B * nn));
+  CHK(dalloc_t(c, &c->tauk, (size_t)B * P * ns * lk * lk / (ns ? 1 : 1)));
+  CHK(dalloc_t(c, &c->trk, (size_t)B * P * ns));
+  std::vector<z*> v(B2);
+  auto fill = [&](PtrArr* pa, auto fn, size_t cnt) -> int {
+    std::vector<z*> t(cnt);
+    for (size_t j = 0; j < cnt; ++j) t[j] = fn(j);
+    return make_ptrs(c, pa, t);
+  };
+  const size_t Bs = B;
+  // decimation chain j: left lead for j < B (ml = M10, mr = M01, tinv = left),
+  // right lead for j >= B (ml = M01, mr = M10, tinv = right)
+  CHK(fill(&c->p_ml, [&](size_t j) { return j < Bs ? c->M10 + j * nn : c->M01 + (j - Bs) * nn; }, B2));
+  CHK(fill(&c->p_mr, [&](size_t j) { return j < Bs ? c->M01 + j * nn : c->M10 + (j - Bs) * nn; }, B2));
+  CHK(fill(&c->p_m0, [&](size_t j) { return c->M00 + (j % Bs) * nn; }, B2));
+  CHK(fill(&c->p_tl, [&](size_t j) { return j < Bs ? c->dTl : c->dTr; }, B2));
+  CHK(fill(&c->p_AB, [&](size_t j) { return c->AB + j * 2 * nn; }, B2));
+  CHK(fill(&c->p_alpha, [&](size_t j) { return c->AB + j * 2 * nn; }, B2));
+  CHK(fill(&c->p_beta, [&](size_t j) { return c->AB + j * 2 * nn + n; }, B2));
+  CHK(fill(&c->p_eps, [&](size_t j) { return c->eps + j * nn; }, B2));
+  CHK(fill(&c->p_x, [&](size_t j) { return c->xs + j * nn; }, B2));
+  CHK(fill(&c->p_Yinv, [&](size_t j) { return c->Yinv + j * nn; }, B2));
+  CHK(fill(&c->p_AYBY, [&](size_t j) { return c->AYBY + j * 2 * nn; }, B2));
+  CHK(fill(&c->p_C1, [&](size_t j) { return c->C1 + j * 2 * nn; }, B2));
+  CHK(fill(&c->p_C2, [&](size_t j) { return c->C2 + j * 2 * nn; }, B2));
+  // self-energy outputs: Y_1 into Ys[0], X_P into Xc
+  CHK(fill(&c->p_sigma, [&](size_t j) { return j < Bs ? c->Ys + j * nn : c->Xc + (j - Bs) * nn; }, B2));
+  CHK(fill(&c->p_M00, [&](size_t j) { return c->M00 + j * nn; }, Bs));
+  CHK(fill(&c->p_M01, [&](size_t j) { return c->M01 + j * nn; }, Bs));
+  CHK(fill(&c->p_M10, [&](size_t j) { return c->M10 + j * nn; }, Bs));
+  CHK(fill(&c->p_W, [&](size_t j) { return c->W + j * nn; }, Bs));
+  CHK(fill(&c->p_Z, [&](size_t j) { return c->Zb + j * nn; }, Bs));
+  CHK(fill(&c->p_X, [&](size_t j) { return c->Xc + j * nn; }, Bs));
+  c->p_Ys.assign(P, PtrArr());
+  c->p_tau.assign(P, PtrArr());
+  c->p_tr.assign(P, PtrArr());
+  for (size_t p = 0; p < P; ++p) {
+    CHK(fill(&c->p_Ys[p], [&](size_t j) { return c->Ys + (p * Bs + j) * nn; }, Bs));
+    CHK(fill(&c->p_tau[p], [&](size_t j) { return c->tauk + (j * P * ns + p * ns) * lk * lk; }, Bs));
+    CHK(fill(&c->p_tr[p], [&](size_t j) { return c->trk + j * P * ns + p * ns; }, Bs));
+  }
+  // sweep Y_1 lives in Ys[0]; p_sigma above points there
+  c->B = B;
+  return SKKR_OK;
+}
+
+// ------------------------------------------------------------------------
+// kernel launchers
+// ------------------------------------------------------------------------
+// Dominant-kernel timing: HIP events recorded on the context stream around
+// each launch (no synchronisation); read back when the stream has drained.
+static void dom_begin(skkr_ctx* c) {
+  if (!c->timing_detail) return;
+  while (c->evpool.size() < c->evused + 2) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    c->evpool.push_back(e);
+  }
+  hipEventRecord(c->evpool[c->evused], c->stream);
+}
+static void dom_end(skkr_ctx* c, double flops) {
+  if (!c->timing_detail) return;
+  hipEventRecord(c->evpool[c->evused + 1], c->stream);
+  c->evflops.push_back(flops);
+  c->evused += 2;
+}
+// call after the stream was synchronised
+static void dom_collect(skkr_ctx* c) {
+  for (size_t i = 0; i + 1 < c->evused; i += 2) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]);
+    c->dom_ms += ms;
+    c->dom_flops += c->evflops[i / 2];
+    c->dom_launches++;
+  }
+  c->evused = 0;
+  c->evflops.clear();
+}
+
+static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
+                        int ldb, z** C, int ldc, z alpha, z beta, int nb,
+                        const int* active) {
+  if (nb <= 0) return SKKR_OK;
+  dom_begin(c);
+  if (m >= 128 && n >= 128) {
+    dim3 grid(((m + 127) / 128) * ((n + 127) / 128), nb);
+    zgemm_f64_kernel<128, 128, 16><<<grid, 1024, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
+  } else if (m > 32 || n > 32) {
+    dim3 grid(((m + 63) / 64) * ((n + 63) / 64), nb);
+    zgemm_f64_kernel<64, 64, 16><<<grid, 256, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
+  } else {
+    dim3 grid(((m + 31) / 32) * ((n + 31) / 32), nb);
+    zgemm_f64_kernel<32, 32, 16><<<grid, 64, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
+  }
+  HIPCHK(hipGetLastError());
+  dom_end(c, 8.0 * m * n * (double)k * nb);
+  c->launches++;
+  return SKKR_OK;
+}
+
+static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
+  if (nb <= 0) return SKKR_OK;
+  if (a.n <= 32) {
+    gjinv_f64_kernel<8, 8, 4, 4><<<nb, 64, 0, c->stream>>>(a);
+  } else if (a.n <= 64) {
+    gjinv_f64_kernel<16, 16, 4, 4><<<nb, 256, 0, c->stream>>>(a);
+  } else if (a.n <= 128) {
+    gjinv_f64_kernel<16, 32, 8, 4><<<nb, 512, 0, c->stream>>>(a);
+  } else {
+    return SKKR_ERR_UNSUPPORTED;
+  }
+  HIPCHK(hipGetLastError());
+  c->launches++;
+  return SKKR_OK;
+}
+
+static InvArgs inv_args(skkr_ctx* c) {
+  InvArgs a;
+  memset(&a, 0, sizeof(a));
+  a.n = c->n;
+  a.lk = c->cfg.lk;
+  return a;
+}
+
+static int launch_fourier(skkr_ctx* c, const z* GR, double sgn, z* out, int nk) {
+  constexpr int KB = 32;
+  dim3 grid((c->nn + 255) / 256, (nk + KB - 1) / KB);
+  if (c->nR <= 16)
+    fourier_kernel<16, KB><<<grid, 256, 0, c->stream>>>(c->nn, c->nR, GR, c->dR, c->dk, nk, sgn, -1.0, out, c->nn);
+  else if (c->nR <= 32)
+    fourier_kernel<32, KB><<<grid, 256, 0, c->stream>>>(c->nn, c->nR, GR, c->dR, c->dk, nk, sgn, -1.0, out, c->nn);
+  else
+    return SKKR_ERR_UNSUPPORTED;
+  HIPCHK(hipGetLastError());
+  c->launches++;
+  return SKKR_OK;
+}
+
+// pl blocks of nk (<= B) k-points already in c->dk
+static int run_fourier(skkr_ctx* c, int nk) {
+  CHK(launch_fourier(c, c->dGRi, -1.0, c->M00, nk));
+  CHK(launch_fourier(c, c->dGRo, -1.0, c->M01, nk));
+  CHK(launch_fourier(c, c->dGRb, +1.0, c->M10, nk));
+  return SKKR_OK;
+}
+
+// Decimation of chains j in [0, 2B) whose active flag the caller set; chains
+// use p_ml/p_mr/p_m0/p_tl.  Leaves g_s = x^-1 in Yinv and the self-energy
+// ml g_s mr in p_sigma when want_sigma.
+static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
+  const int n = c->n, B2 = 2 * c->B;
+  dec_init_kernel<<<B2, 256, 0, c->stream>>>(n, c->cfg.lk, c->p_ml.d, c->p_mr.d, c->p_m0.d,
+                                             add_tinv ? c->p_tl.d : nullptr, c->p_AB.d,
+                                             c->p_eps.d, c->p_x.d);
+  HIPCHK(hipGetLastError());
+  // chains never active are skipped by the kernels; keep a copy of the mask
+  std::vector<int> act(B2);
+  HIPCHK(hipMemcpyAsync(act.data(), c->active, B2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int itermax = std::max(c->cfg.itermax_l, c->cfg.itermax_r);
+  for (int iter = 1; iter <= itermax + 1; ++iter) {
+    InvArgs ia = inv_args(c);
+    ia.base = c->p_eps.d;
+    ia.out = c->p_Yinv.d;
+    ia.active = c->active;
+    ia.status = c->status;
+    CHK(launch_gjinv(c, ia, B2));
+    const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB.d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
+                     one, zero, B2, c->active));
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha.d, 2 * n, c->p_C1.d, 2 * n,
+                     one, zero, B2, c->active));
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta.d, 2 * n, c->p_C2.d, 2 * n,
+                     one, zero, B2, c->active));
+    HIPCHK(hipMemsetAsync(c->nactive, 0, sizeof(int), c->stream));
+    dec_update_kernel<<<B2, 256, 0, c->stream>>>(n, iter, c->cfg.itermax_l, c->cfg.itermax_r,
+                                                 c->B, c->cfg.errmax, c->p_C1.d,
+                                                 c->p_C2.d, c->p_AB.d, c->p_eps.d, c->p_x.d,
+                                                 c->active, c->iters, c->status, c->nactive);
+    HIPCHK(hipGetLastError());
+    c->launches += 2;
+    int nact = 0;
+    HIPCHK(hipMemcpyAsync(&nact, c->nactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (nact == 0) break;
+  }
+  // g_s = x^-1 for every chain that was started
+  HIPCHK(hipMemcpyAsync(c->active, act.data(), B2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  InvArgs ia = inv_args(c);
+  ia.base = c->p_x.d;
+  ia.out = c->p_Yinv.d;
+  ia.active = c->active;
+  ia.status = c->status;
+  CHK(launch_gjinv(c, ia, B2));
+  if (want_sigma) {
+    const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+    // T = ml g_s (into AYBY as n x n), sigma = T mr
+    CHK(launch_zgemm(c, n, n, n, c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero, B2,
+                     c->active));
+    CHK(launch_zgemm(c, n, n, n, c->p_AYBY.d, n, c->p_mr.d, n, c->p_sigma.d, n, one, zero, B2,
+                     c->active));
+  }
+  return SKKR_OK;
+}
+
+// Godfrin sweeps for nb k-points: Y_1 in Ys[0], X_P in Xc, blocks in M00/M01/M10.
+static int run_sweeps(skkr_ctx* c, int nb) {
+  const int n = c->n, P = c->cfg.n_pl, s = c->cfg.sites_per_pl, lk = c->cfg.lk;
+  const size_t tblk = (size_t)s * lk * lk;
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // Y sweep (tau2d.f:53-85)
+  for (int p = 0; p < P - 1; ++p) {
+    InvArgs ia = inv_args(c);
+    ia.base = c->p_M00.d;
+    ia.tinv = c->dTint + p * tblk;
+    ia.sub1 = c->p_Ys[p].d;
+    ia.out = c->p_W.d;
+    ia.status = c->status_sw;
+    CHK(launch_gjinv(c, ia, nb));
+    CHK(launch_zgemm(c, n, n, n, c->p_W.d, n, c->p_M01.d, n, c->p_Z.d, n, one, zero, nb, nullptr));
+    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_Z.d, n, c->p_Ys[p + 1].d, n, one, zero, nb,
+                     nullptr));
+  }
+  // X sweep (tau2d.f:89-109) with the diagonal blocks (:114-149) fused
+  for (int p = P - 1; p >= 0; --p) {
+    if (p < P - 1)
+      CHK(launch_zgemm(c, n, n, n, c->p_M01.d, n, c->p_Z.d, n, c->p_X.d, n, one, zero, nb,
+                       nullptr));
+    InvArgs ia = inv_args(c);
+    ia.base = c->p_M00.d;
+    ia.tinv = c->dTint + p * tblk;
+    ia.sub1 = c->p_X.d;
+    ia.sub2 = c->p_Ys[p].d;
+    ia.tau = c->p_tau[p].d;
+    ia.tr = c->p_tr[p].d;
+    ia.status = c->status_sw;
+    ia.mode = 1;
+    CHK(launch_gjinv(c, ia, nb));
+    if (p > 0) {
+      InvArgs ib = inv_args(c);
+      ib.base = c->p_M00.d;
+      ib.tinv = c->dTint + p * tblk;
+      ib.sub1 = c->p_X.d;
+      ib.out = c->p_W.d;
+      ib.status = c->status_sw;
+      CHK(launch_gjinv(c, ib, nb));
+      CHK(launch_zgemm(c, n, n, n, c->p_W.d, n, c->p_M10.d, n, c->p_Z.d, n, one, zero, nb,
+                       nullptr));
+    }
+  }
+  return SKKR_OK;
+}
+
+static int status_code(int bits) {
+  if (bits & 4) return SKKR_ERR_SGF_ITERMAX;
+  if (bits & 2) return SKKR_ERR_SGF_DIVERGED;
+  if (bits & 1) return SKKR_ERR_SINGULAR;
+  return SKKR_OK;
+}
+
+// ------------------------------------------------------------------------
+// C-ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+const char* skkr_error_string(int code) {
+  switch (code) {
+    case SKKR_OK: return "ok";
+    case SKKR_ERR_ARG: return "invalid argument";
+    case SKKR_ERR_HIP: return "HIP runtime error";
+    case SKKR_ERR_NOMEM: return "device out of memory";
+    case SKKR_ERR_SINGULAR: return "singular pivot";
+    case SKKR_ERR_SGF_DIVERGED: return "decimation diverged";
+    case SKKR_ERR_SGF_ITERMAX: return "decimation exceeded itermax";
+    case SKKR_ERR_UNSUPPORTED: return "unsupported size or mode";
+  }
+  return "unknown error";
+}
+
+int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
+  if (!out || !cfg) return SKKR_ERR_ARG;
+  *out = nullptr;
+  if (cfg->lk <= 0 || cfg->sites_per_pl <= 0 || cfg->n_pl <= 0 || cfg->errmax <= 0 ||
+      cfg->itermax_l < 1 || cfg->itermax_r < 1)
+    return SKKR_ERR_ARG;
+  if (cfg->precision != SKKR_FP64) return SKKR_ERR_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
+    return SKKR_ERR_HIP;
+  HIPCHK(hipSetDevice(cfg->device));
+  skkr_ctx* c = new skkr_ctx();
+  c->cfg = *cfg;
+  c->n = cfg->lk * cfg->sites_per_pl;
+  c->nn = c->n * c->n;
+  c->ns = cfg->sites_per_pl;
+  if (c->n > 128) {
+    delete c;
+    return SKKR_ERR_UNSUPPORTED;
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SKKR_ERR_HIP;
+  }
+  const size_t tsz = (size_t)cfg->lk * cfg->lk;
+  int r = SKKR_OK;
+  if ((r = dalloc_t(c, &c->dTint, tsz * cfg->n_pl * cfg->sites_per_pl)) != SKKR_OK ||
+      (r = dalloc_t(c, &c->dTl, tsz * cfg->sites_per_pl)) != SKKR_OK ||
+      (r = dalloc_t(c, &c->dTr, tsz * cfg->sites_per_pl)) != SKKR_OK ||
+      (r = dalloc_t(c, &c->taubz, tsz * cfg->n_pl * cfg->sites_per_pl)) != SKKR_OK) {
+    skkr_destroy(c);
+    return r;
+  }
+  *out = c;
+  return SKKR_OK;
+}
+
+void skkr_destroy(skkr_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->cfg.device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+  delete c;
+}
+
+int skkr_set_structure(skkr_ctx* c, int nR, const double* R, const double* GRi,
+                       const double* GRo) {
+  if (!c || nR <= 0 || nR > 32 || !R || !GRi || !GRo) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (c->have_structure && nR != c->nR) return SKKR_ERR_ARG;
+  const int n = c->n;
+  const size_t nn = c->nn;
+  if (!c->have_structure) {
+    c->nR = nR;
+    CHK(dalloc_t(c, &c->dR, 2 * (size_t)nR));
+    CHK(dalloc_t(c, &c->dGRi, nR * nn));
+    CHK(dalloc_t(c, &c->dGRo, nR * nn));
+    CHK(dalloc_t(c, &c->dGRb, nR * nn));
+  }
+  // back coupling: G_inter(R)^dagger for every R, summed with e^{+ik.R}
+  std::vector<double> back(2 * nR * nn);
+  for (int r = 0; r < nR; ++r)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        size_t src = (size_t)r * nn + i + (size_t)j * n;
+        size_t dst = (size_t)r * nn + j + (size_t)i * n;
+        back[2 * dst] = GRo[2 * src];
+        back[2 * dst + 1] = -GRo[2 * src + 1];
+      }
+  HIPCHK(hipMemcpy(c->dR, R, 2 * sizeof(double) * nR, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dGRi, GRi, sizeof(z) * nR * nn, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dGRo, GRo, sizeof(z) * nR * nn, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dGRb, back.data(), sizeof(z) * nR * nn, hipMemcpyHostToDevice));
+  c->have_structure = true;
+  return SKKR_OK;
+}
+
+int skkr_set_tinv(skkr_ctx* c, const double* ti, const double* tl, const double* tr) {
+  if (!c || !ti || !tl || !tr) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t tsz = (size_t)c->cfg.lk * c->cfg.lk;
+  HIPCHK(hipMemcpy(c->dTint, ti, sizeof(z) * tsz * c->cfg.n_pl * c->ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dTl, tl, sizeof(z) * tsz * c->ns, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dTr, tr, sizeof(z) * tsz * c->ns, hipMemcpyHostToDevice));
+  c->have_tinv = true;
+  return SKKR_OK;
+}
+
+static int auto_batch(skkr_ctx* c, int nk) {
+  int B = c->cfg.batch;
+  if (B <= 0) {
+    // ~ 22 n^2 + P n^2 complex per k; keep a batch under ~24 GB
+    double per_k = (22.0 + c->cfg.n_pl) * c->nn * 16.0;
+    B = (int)std::min(1024.0, std::max(1.0, 24e9 / per_k));
+  }
+  return std::max(1, std::min(B, nk));
+}
+
+int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
+                const double* w, double* tau_bz, double* tr_tau_k, int* iters_k,
+                int* status_k, skkr_stats* st) {
+  (void)e_re;
+  (void)e_im;
+  if (!c || nk < 0 || (nk > 0 && (!k || !w)) || !tau_bz) return SKKR_ERR_ARG;
+  if (!c->have_structure || !c->have_tinv) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  auto t0 = std::chrono::steady_clock::now();
+  const int B = auto_batch(c, std::max(nk, 1));
+  CHK(ensure_batch(c, B));
+  const int P = c->cfg.n_pl, lk = c->cfg.lk, ns = c->ns;
+  const size_t sites = (size_t)P * ns, tsz = sites * lk * lk;
+  c->timing_detail = st ? st->timing_detail : 0;
+  c->dom_ms = 0;
+  c->dom_flops = 0;
+  c->dom_launches = 0;
+  c->evused = 0;
+  c->evflops.clear();
+  c->launches = 0;
+  HIPCHK(hipMemsetAsync(c->taubz, 0, sizeof(z) * tsz, c->stream));
+  hipEvent_t ev[5];
+  for (auto& e : ev) hipEventCreate(&e);
+  double ms_f = 0, ms_d = 0, ms_s = 0, ms_a = 0;
+  double sum_l = 0, sum_r = 0;
+  int maxit = 0, nbatches = 0;
+  int rc = SKKR_OK;
+  std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B);
+  std::vector<double> h_tr(2 * sites * B);
+  for (int k0 = 0; k0 < nk; k0 += B) {
+    const int nb = std::min(B, nk - k0);
+    ++nbatches;
+    HIPCHK(hipMemcpyAsync(c->dk, k + 2 * (size_t)k0, 2 * sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dw, w + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    for (int j = 0; j < 2 * B; ++j) act[j] = (j % B) < nb ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->status_sw, 0, B * sizeof(int), c->stream));
+    hipEventRecord(ev[0], c->stream);
+    CHK(run_fourier(c, nb));
+    hipEventRecord(ev[1], c->stream);
+    CHK(run_decimation(c, true, true));
+    hipEventRecord(ev[2], c->stream);
+    CHK(run_sweeps(c, nb));
+    hipEventRecord(ev[3], c->stream);
+    {
+      int ne = (int)tsz;
+      bz_accumulate_kernel<<<(ne + 255) / 256, 256, 0, c->stream>>>(ne, nb, c->tauk, (long long)tsz, c->dw, c->taubz);
+      HIPCHK(hipGetLastError());
+      c->launches++;
+    }
+    hipEventRecord(ev[4], c->stream);
+    HIPCHK(hipMemcpyAsync(h_it.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_st.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_sw.data(), c->status_sw, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (tr_tau_k)
+      HIPCHK(hipMemcpyAsync(tr_tau_k + 2 * sites * (size_t)k0, c->trk, sizeof(z) * sites * nb,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dom_collect(c);
+    float t;
+    hipEventElapsedTime(&t, ev[0], ev[1]); ms_f += t;
+    hipEventElapsedTime(&t, ev[1], ev[2]); ms_d += t;
+    hipEventElapsedTime(&t, ev[2], ev[3]); ms_s += t;
+    hipEventElapsedTime(&t, ev[3], ev[4]); ms_a += t;
+    for (int b = 0; b < nb; ++b) {
+      int il = h_it[b], ir = h_it[B + b];
+      sum_l += il;
+      sum_r += ir;
+      maxit = std::max(maxit, std::max(il, ir));
+      int code = status_code(h_st[b] | h_st[B + b] | h_sw[b]);
+      if (iters_k) {
+        iters_k[2 * (size_t)(k0 + b)] = il;
+        iters_k[2 * (size_t)(k0 + b) + 1] = ir;
+      }
+      if (status_k) status_k[k0 + b] = code;
+      if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+    }
+  }
+  std::vector<double> hbz(2 * tsz);
+  HIPCHK(hipMemcpyAsync(hbz.data(), c->taubz, sizeof(z) * tsz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t e = 0; e < 2 * tsz; ++e) tau_bz[e] += hbz[e];
+  for (auto& e : ev) hipEventDestroy(e);
+  if (st) {
+    st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st->ms_fourier = ms_f;
+    st->ms_decimation = ms_d;
+    st->ms_sweeps = ms_s;
+    st->ms_accumulate = ms_a;
+    st->ms_refine = 0;
+    st->mean_iter_l = nk ? sum_l / nk : 0;
+    st->mean_iter_r = nk ? sum_r / nk : 0;
+    st->max_iter = maxit;
+    st->n_batches = nbatches;
+    st->n_launches = c->launches;
+    st->dom_launches = c->dom_launches;
+    st->dom_ms = c->dom_ms;
+    st->dom_flops = c->dom_flops;
+    snprintf(st->dom_name, sizeof(st->dom_name), "zgemm_f64_kernel");
+  }
+  c->timing_detail = 0;
+  return rc;
+}
+
+// ---------------- per-routine batched entry points ----------------------
+int skkr_pl_blocks(skkr_ctx* c, int nk, const double* k, double* m00, double* m01,
+                   double* m10) {
+  if (!c || nk <= 0 || !k || !m00 || !m01 || !m10 || !c->have_structure) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t nn = c->nn;
+  for (int k0 = 0; k0 < nk; k0 += 256) {
+    int nb = std::min(256, nk - k0);
+    CHK(ensure_batch(c, std::max(c->B, nb)));
+    HIPCHK(hipMemcpyAsync(c->dk, k + 2 * (size_t)k0, 2 * sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    CHK(run_fourier(c, nb));
+    HIPCHK(hipMemcpyAsync(m00 + 2 * nn * k0, c->M00, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(m01 + 2 * nn * k0, c->M01, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(m10 + 2 * nn * k0, c->M10, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SKKR_OK;
+}
+
+// temporary device copy of nb host matrices + pointer array
+struct TmpMats {
+  z* d = nullptr;
+  z** p = nullptr;
+  ~TmpMats() {
+    if (d) hipFree(d);
+    if (p) hipFree(p);
+  }
+  int init(int nb, size_t nn, const double* host, hipStream_t s) {
+    if (hipMalloc(&d, sizeof(z) * nn * nb) != hipSuccess) return SKKR_ERR_NOMEM;
+    if (hipMalloc(&p, sizeof(z*) * nb) != hipSuccess) return SKKR_ERR_NOMEM;
+    std::vector<z*> v(nb);
+    for (int j = 0; j < nb; ++j) v[j] = d + j * nn;
+    HIPCHK(hipMemcpy(p, v.data(), sizeof(z*) * nb, hipMemcpyHostToDevice));
+    if (host) HIPCHK(hipMemcpyAsync(d, host, sizeof(z) * nn * nb, hipMemcpyHostToDevice, s));
+    return SKKR_OK;
+  }
+};
+
+int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
+  if (!c || n <= 0 || n > 128 || nb <= 0 || !a) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  TmpMats m, st;
+  CHK(m.init(nb, (size_t)n * n, a, c->stream));
+  int* dst = nullptr;
+  HIPCHK(hipMalloc(&dst, sizeof(int) * nb));
+  HIPCHK(hipMemsetAsync(dst, 0, sizeof(int) * nb, c->stream));
+  InvArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = n;
+  ia.lk = n;
+  ia.base = m.p;
+  ia.out = m.p;
+  ia.status = dst;
+  int r = launch_gjinv(c, ia, nb);
+  std::vector<int> hs(nb);
+  if (r == SKKR_OK) {
+    HIPCHK(hipMemcpyAsync(a, m.d, sizeof(z) * n * n * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < nb; ++j) {
+      int code = status_code(hs[j]);
+      if (status) status[j] = code;
+      if (code != SKKR_OK && r == SKKR_OK) r = code;
+    }
+  }
+  hipFree(dst);
+  return r;
+}
+
+int skkr_zgemm_batch(skkr_ctx* c, int n, int nb, const double* a, const double* b, double* cc) {
+  if (!c || n <= 0 || nb <= 0 || !a || !b || !cc) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  TmpMats A, Bm, C;
+  const size_t nn = (size_t)n * n;
+  CHK(A.init(nb, nn, a, c->stream));
+  CHK(Bm.init(nb, nn, b, c->stream));
+  CHK(C.init(nb, nn, nullptr, c->stream));
+  CHK(launch_zgemm(c, n, n, n, A.p, n, Bm.p, n, C.p, n, make_double2(1, 0), make_double2(0, 0), nb, nullptr));
+  HIPCHK(hipMemcpyAsync(cc, C.d, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SKKR_OK;
+}
+
+// decimation of nb chains given by host matrices; lead t^-1 added for dugo
+static int dec_host(skkr_ctx* c, int nb, const double* ml, const double* m0, const double* mr,
+                    bool add_tinv, char side, double* x_out, double* sigma_out, int* iters,
+                    int* status) {
+  const size_t nn = c->nn;
+  CHK(ensure_batch(c, std::max(c->B, nb)));
+  const int B = c->B;
+  // chains occupy the left half [0, nb) (or the right half for side 'R' so that
+  // the right lead t^-1 is used)
+  const int off = (side == 'R') ? B : 0;
+  z* mlbuf = (side == 'R') ? c->M01 : c->M10;
+  z* mrbuf = (side == 'R') ? c->M10 : c->M01;
+  HIPCHK(hipMemcpyAsync(mlbuf, ml, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(mrbuf, mr, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->M00, m0, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  std::vector<int> act(2 * B, 0);
+  for (int j = 0; j < nb; ++j) act[off + j] = 1;
+  HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+  CHK(run_decimation(c, add_tinv, sigma_out != nullptr));
+  if (x_out)
+    HIPCHK(hipMemcpyAsync(x_out, c->Yinv + off * nn, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+  if (sigma_out) {
+    z* src = (side == 'R') ? c->Xc : c->Ys;
+    HIPCHK(hipMemcpyAsync(sigma_out, src, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
+  }
+  std::vector<int> hi(2 * B), hs(2 * B);
+  HIPCHK(hipMemcpyAsync(hi.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hs.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = SKKR_OK;
+  for (int j = 0; j < nb; ++j) {
+    int code = status_code(hs[off + j]);
+    if (iters) iters[j] = hi[off + j];
+    if (status) status[j] = code;
+    if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+  }
+  return rc;
+}
+
+int skkr_surfgfd_batch(skkr_ctx* c, int nb, const double* ml, const double* m0,
+                       const double* mr, double* x, int* iters, int* status) {
+  if (!c || nb <= 0 || !ml || !m0 || !mr || !x) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  return dec_host(c, nb, ml, m0, mr, false, 'L', x, nullptr, iters, status);
+}
+
+int skkr_dugo_batch(skkr_ctx* c, char side, int nb, const double* m00, const double* m01,
+                    const double* m10, double* xy, int* iters, int* status) {
+  if (!c || nb <= 0 || !m00 || !m01 || !m10 || !xy || (side != 'L' && side != 'R') || !c->have_tinv)
+    return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (side == 'L') return dec_host(c, nb, m10, m00, m01, true, 'L', nullptr, xy, iters, status);
+  return dec_host(c, nb, m01, m00, m10, true, 'R', nullptr, xy, iters, status);
+}
+
+int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
+                     const double* m10, const double* x, const double* y, double w,
+                     double* tau, int* status) {
+  if (!c || nb <= 0 || !m00 || !m01 || !m10 || !x || !y || !tau || !c->have_tinv)
+    return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  CHK(ensure_batch(c, std::max(c->B, nb)));
+  const size_t nn = c->nn;
+  const int P = c->cfg.n_pl, lk = c->cfg.lk;
+  const size_t tsz = (size_t)P * c->ns * lk * lk;
+  HIPCHK(hipMemcpyAsync(c->M00, m00, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->M01, m01, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->M10, m10, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Xc, x, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Ys, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
+  CHK(run_sweeps(c, nb));
+  long long ne = (long long)tsz * nb;
+  scale_copy_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(ne, c->tauk, w, c->tauk);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tau, c->tauk, sizeof(z) * tsz * nb, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> hs(nb);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->status_sw, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = SKKR_OK;
+  for (int j = 0; j < nb; ++j) {
+    int code = status_code(hs[j]);
+    if (status) status[j] = code;
+    if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+  }
+  return rc;
+}
+
+}  // extern "C"

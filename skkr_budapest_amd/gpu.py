@@ -1,0 +1,233 @@
+"""ctypes binding of libskkr_gpu.so (the C-ABI declared in include/skkr_gpu.h).
+
+``Context`` mirrors the reference's k-loop contract (cpacoord's k-loop body,
+skkr_kpar_v6/rsp-scf/cpacoord_ksave.f:188-400): set the structure constants
+and t^-1, then ``tau_bz`` accumulates the BZ-integrated site-diagonal tau.
+The per-routine methods (``gjinv``, ``surfgfd``, ``dugo``, ``tau2d``,
+``pl_blocks``, ``zgemm``) stand for gjinv.f, surfgf.f, dugo.f, tau2d.f,
+gstore+packer and doubmt2 and raise on the error codes where the reference
+would stop.
+
+There is no CPU fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .systems import colmajor, from_colmajor
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libskkr_gpu.so")
+
+SKKR_OK = 0
+ERRORS = {1: "SKKR_ERR_ARG", 2: "SKKR_ERR_HIP", 3: "SKKR_ERR_NOMEM",
+          4: "SKKR_ERR_SINGULAR", 5: "SKKR_ERR_SGF_DIVERGED",
+          6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED"}
+FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
+
+# exported symbols (every declaration of include/skkr_gpu.h)
+SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structure",
+           "skkr_set_tinv", "skkr_tau_bz", "skkr_pl_blocks", "skkr_gjinv_batch",
+           "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
+           "skkr_tau2d_batch")
+
+
+class SkkrError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {ERRORS.get(code, code)}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("lk", ctypes.c_int), ("sites_per_pl", ctypes.c_int),
+                ("n_pl", ctypes.c_int), ("errmax", ctypes.c_double),
+                ("itermax_l", ctypes.c_int), ("itermax_r", ctypes.c_int),
+                ("precision", ctypes.c_int), ("fp64_dec_below_imE", ctypes.c_double),
+                ("device", ctypes.c_int), ("batch", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_fourier", ctypes.c_double),
+                ("ms_decimation", ctypes.c_double), ("ms_sweeps", ctypes.c_double),
+                ("ms_accumulate", ctypes.c_double), ("ms_refine", ctypes.c_double),
+                ("mean_iter_l", ctypes.c_double), ("mean_iter_r", ctypes.c_double),
+                ("max_iter", ctypes.c_int), ("n_batches", ctypes.c_int),
+                ("n_launches", ctypes.c_longlong), ("timing_detail", ctypes.c_int),
+                ("dom_launches", ctypes.c_longlong), ("dom_ms", ctypes.c_double),
+                ("dom_flops", ctypes.c_double), ("dom_name", ctypes.c_char * 32)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_}
+        d["dom_name"] = self.dom_name.decode()
+        return d
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        V, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.skkr_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(Config)]
+        L.skkr_destroy.argtypes = [V]
+        L.skkr_destroy.restype = None
+        L.skkr_error_string.argtypes = [I]
+        L.skkr_error_string.restype = ctypes.c_char_p
+        L.skkr_set_structure.argtypes = [V, I, V, V, V]
+        L.skkr_set_tinv.argtypes = [V, V, V, V]
+        L.skkr_tau_bz.argtypes = [V, D, D, I, V, V, V, V, V, V, ctypes.POINTER(Stats)]
+        L.skkr_pl_blocks.argtypes = [V, I, V, V, V, V]
+        L.skkr_gjinv_batch.argtypes = [V, I, I, V, V]
+        L.skkr_zgemm_batch.argtypes = [V, I, I, V, V, V]
+        L.skkr_surfgfd_batch.argtypes = [V, I, V, V, V, V, V, V]
+        L.skkr_dugo_batch.argtypes = [V, ctypes.c_char, I, V, V, V, V, V, V]
+        L.skkr_tau2d_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, np.complex128)
+
+
+class Context:
+    """One device context (skkr_create ... skkr_destroy)."""
+
+    def __init__(self, lk, sites_per_pl, n_pl, errmax=1e-8, itermax=50,
+                 precision=FP64, fp64_dec_below_imE=0.02, device=0, batch=0):
+        self.lk, self.s, self.P = lk, sites_per_pl, n_pl
+        self.n = lk * sites_per_pl
+        cfg = Config(lk, sites_per_pl, n_pl, errmax, itermax, itermax, precision,
+                     fp64_dec_below_imE, device, batch)
+        h = ctypes.c_void_p()
+        self._check(lib().skkr_create(ctypes.byref(h), ctypes.byref(cfg)), "skkr_create")
+        self._h = h
+
+    @classmethod
+    def for_system(cls, system, **kw):
+        c = system.cfg
+        kw.setdefault("errmax", c.errmax)
+        kw.setdefault("itermax", c.itermax)
+        ctx = cls(c.lk, c.s, c.P, **kw)
+        ctx.set_structure(system.R, system.G_intra, system.G_inter)
+        return ctx
+
+    @staticmethod
+    def _check(rc, what):
+        if rc != SKKR_OK:
+            raise SkkrError(rc, what)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().skkr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- k-loop -----------------------------------------------------------
+    def set_structure(self, R, G_intra, G_inter):
+        R = np.ascontiguousarray(R, np.float64)
+        self._gi = colmajor(_c128(G_intra))
+        self._go = colmajor(_c128(G_inter))
+        self._check(lib().skkr_set_structure(self._h, len(R), _p(R), _p(self._gi), _p(self._go)),
+                    "skkr_set_structure")
+
+    def set_tinv(self, tinv_int, tinv_l, tinv_r):
+        a, b, c = (colmajor(_c128(t)) for t in (tinv_int, tinv_l, tinv_r))
+        self._check(lib().skkr_set_tinv(self._h, _p(a), _p(b), _p(c)), "skkr_set_tinv")
+
+    def tau_bz(self, E, k, w, tau_bz=None, want_trk=True, timing_detail=False,
+               raise_on_error=True):
+        k = np.ascontiguousarray(k, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        nk = len(w)
+        ns = self.P * self.s
+        acc = np.zeros((ns, self.lk, self.lk), np.complex128) if tau_bz is None else tau_bz
+        buf = colmajor(acc)
+        trk = np.zeros((nk, ns), np.complex128) if want_trk else None
+        iters = np.zeros((nk, 2), np.int32)
+        status = np.zeros(nk, np.int32)
+        st = Stats()
+        st.timing_detail = 1 if timing_detail else 0
+        rc = lib().skkr_tau_bz(self._h, float(np.real(E)), float(np.imag(E)), nk, _p(k), _p(w),
+                               _p(buf), _p(trk), _p(iters), _p(status), ctypes.byref(st))
+        if raise_on_error:
+            self._check(rc, "skkr_tau_bz")
+        acc[...] = from_colmajor(buf)
+        return {"tau_bz": acc, "trk": trk, "iters": iters, "status": status,
+                "stats": st.as_dict(), "rc": rc}
+
+    # ---- per-routine entry points ----------------------------------------
+    def pl_blocks(self, k):
+        k = np.ascontiguousarray(k, np.float64)
+        nk = len(k)
+        out = [np.zeros((nk, self.n, self.n), np.complex128) for _ in range(3)]
+        self._check(lib().skkr_pl_blocks(self._h, nk, _p(k), *map(_p, out)), "skkr_pl_blocks")
+        return tuple(from_colmajor(o) for o in out)
+
+    def gjinv(self, a):
+        a = np.asarray(a)
+        nb, n = a.shape[0], a.shape[1]
+        buf = colmajor(_c128(a))
+        status = np.zeros(nb, np.int32)
+        rc = lib().skkr_gjinv_batch(self._h, n, nb, _p(buf), _p(status))
+        if rc not in (SKKR_OK, 4):
+            self._check(rc, "skkr_gjinv_batch")
+        return from_colmajor(buf), status
+
+    def zgemm(self, a, b):
+        nb, n = a.shape[0], a.shape[1]
+        A, B = colmajor(_c128(a)), colmajor(_c128(b))
+        C = np.zeros_like(A)
+        self._check(lib().skkr_zgemm_batch(self._h, n, nb, _p(A), _p(B), _p(C)), "skkr_zgemm_batch")
+        return from_colmajor(C)
+
+    def surfgfd(self, ml, m0, mr):
+        nb = ml.shape[0]
+        bufs = [colmajor(_c128(m)) for m in (ml, m0, mr)]
+        x = np.zeros_like(bufs[0])
+        iters = np.zeros(nb, np.int32)
+        status = np.zeros(nb, np.int32)
+        lib().skkr_surfgfd_batch(self._h, nb, *map(_p, bufs), _p(x), _p(iters), _p(status))
+        return from_colmajor(x), iters, status
+
+    def dugo(self, side, m00, m01, m10):
+        nb = m00.shape[0]
+        bufs = [colmajor(_c128(m)) for m in (m00, m01, m10)]
+        xy = np.zeros_like(bufs[0])
+        iters = np.zeros(nb, np.int32)
+        status = np.zeros(nb, np.int32)
+        lib().skkr_dugo_batch(self._h, side.encode(), nb, *map(_p, bufs), _p(xy), _p(iters),
+                              _p(status))
+        return from_colmajor(xy), iters, status
+
+    def tau2d(self, m00, m01, m10, x, y, w=1.0):
+        nb = m00.shape[0]
+        bufs = [colmajor(_c128(m)) for m in (m00, m01, m10, x, y)]
+        ns = self.P * self.s
+        tau = np.zeros((nb, ns, self.lk, self.lk), np.complex128)
+        tb = colmajor(tau)
+        status = np.zeros(nb, np.int32)
+        lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
+        return from_colmajor(tb), status

@@ -1,0 +1,47 @@
+"""The C-ABI library: it is built, loads without a GPU and exports every
+function include/skkr_gpu.h declares (no compute calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "skkr_gpu.h")
+LIB = os.path.join(ROOT, "skkr_budapest_amd", "libskkr_gpu.so")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(skkr_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_k_loop_entry_points():
+    d = declared()
+    for name in ("skkr_create", "skkr_destroy", "skkr_set_structure", "skkr_set_tinv",
+                 "skkr_tau_bz", "skkr_dugo_batch", "skkr_tau2d_batch",
+                 "skkr_surfgfd_batch", "skkr_gjinv_batch"):
+        assert name in d
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [s for s in declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_python_binding_lists_every_symbol():
+    from skkr_budapest_amd import gpu
+    assert sorted(gpu.SYMBOLS) == declared()
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_argument_validation_without_device():
+    lib = ctypes.CDLL(LIB)
+    lib.skkr_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.skkr_create(None, None) == 1          # SKKR_ERR_ARG
+    lib.skkr_error_string.restype = ctypes.c_char_p
+    assert lib.skkr_error_string(6) == b"decimation exceeded itermax"

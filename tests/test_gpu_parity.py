@@ -1,0 +1,213 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle and the
+reference's golden vectors.  FP64 tolerance: relative Frobenius error
+<= 1e-11 for tau (round-off of ~1e3 dependent complex operations per k),
+identical decimation iteration counts."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from skkr_budapest_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-11
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+def small_sys(lk=4, s=2, P=3, tmodel="herm", E=0.3 + 0.1j, r2max=2):
+    cfg = S.small_config(lk=lk, s=s, P=P, r2max=r2max, tmodel=tmodel, energies=(E,))
+    return S.make_system(cfg)
+
+
+@pytest.mark.parametrize("n", [9, 18, 36, 64, 128])
+def test_zgemm_matches_numpy(gpu, n):
+    rng = np.random.default_rng(n)
+    a = rng.normal(size=(5, n, n)) + 1j * rng.normal(size=(5, n, n))
+    b = rng.normal(size=(5, n, n)) + 1j * rng.normal(size=(5, n, n))
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        c = ctx.zgemm(a, b)
+    ref = a @ b
+    assert rel(c, ref) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 9, 18, 36, 64, 100, 128])
+def test_gjinv_matches_numpy(gpu, n):
+    rng = np.random.default_rng(100 + n)
+    a = rng.normal(size=(7, n, n)) + 1j * rng.normal(size=(7, n, n))
+    a[0, 0, 0] = 1e-14  # forces pivoting
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        inv, st = ctx.gjinv(a)
+        assert (st == 0).all()
+        res = np.abs(inv @ a - np.eye(n)).max()
+        assert res < 1e-9
+        z, st = ctx.gjinv(np.zeros((2, n, n)))
+        assert (st == 4).all()
+
+
+def test_pl_blocks_match_oracle(gpu):
+    sy = S.make_system(S.small_config(lk=6, s=3, P=2, r2max=8))
+    rng = np.random.default_rng(3)
+    k = rng.uniform(-np.pi, np.pi, size=(40, 2))
+    with gpu.Context.for_system(sy) as ctx:
+        m00, m01, m10 = ctx.pl_blocks(k)
+    for i in range(0, 40, 7):
+        gi = oracle.fourier(sy.R, sy.G_intra, k[i])
+        go = oracle.fourier(sy.R, sy.G_inter, k[i])
+        r00, r01, r10 = oracle.pl_blocks(gi, go)
+        for a, b in ((m00[i], r00), (m01[i], r01), (m10[i], r10)):
+            assert np.abs(a - b).max() < 1e-14 * max(1.0, np.abs(b).max())
+
+
+def _blocks(sy, k):
+    gi = oracle.fourier(sy.R, sy.G_intra, k)
+    go = oracle.fourier(sy.R, sy.G_inter, k)
+    return oracle.pl_blocks(gi, go)
+
+
+def test_surfgfd_and_dugo_match_oracle(gpu):
+    sy = small_sys(lk=5, s=2, P=2)
+    E = sy.cfg.energies[0]
+    ti, tl, tr = sy.tinv(E)
+    ks = np.array([[0.1, 0.2], [-1.0, 2.5], [3.0, -0.4]])
+    B = [_blocks(sy, k) for k in ks]
+    m00 = np.stack([b[0] for b in B]); m01 = np.stack([b[1] for b in B])
+    m10 = np.stack([b[2] for b in B])
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(ti, tl, tr)
+        # plain surfgfd on m00 + t_left
+        m0 = m00.copy()
+        for i in range(3):
+            full = m00[i].copy()
+            for a in range(sy.s):
+                full[a * sy.lk:(a + 1) * sy.lk, a * sy.lk:(a + 1) * sy.lk] += tl[a]
+            m0[i] = full
+        x, it, st = ctx.surfgfd(m10, m0, m01)
+        for i in range(3):
+            xr, itr, str_ = oracle.surfgfd(m10[i], m0[i], m01[i])
+            assert it[i] == itr and st[i] == 0
+            assert rel(x[i], xr) < TOL
+        for side in ("L", "R"):
+            xy, it, st = ctx.dugo(side, m00, m01, m10)
+            for i in range(3):
+                ref, itr, _ = oracle.dugo(m00[i], m01[i], m10[i], tl if side == "L" else tr, side)
+                assert it[i] == itr and st[i] == 0
+                assert rel(xy[i], ref) < TOL
+
+
+def test_tau2d_matches_oracle(gpu):
+    sy = small_sys(lk=4, s=3, P=5)
+    E = sy.cfg.energies[0]
+    ti, tl, tr = sy.tinv(E)
+    ks = np.array([[0.3, -0.2], [1.7, 0.9]])
+    B = [_blocks(sy, k) for k in ks]
+    m00 = np.stack([b[0] for b in B]); m01 = np.stack([b[1] for b in B])
+    m10 = np.stack([b[2] for b in B])
+    X = np.stack([oracle.dugo(*B[i], tr, "R")[0] for i in range(2)])
+    Y = np.stack([oracle.dugo(*B[i], tl, "L")[0] for i in range(2)])
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(ti, tl, tr)
+        tau, st = ctx.tau2d(m00, m01, m10, X, Y, w=0.5)
+    assert (st == 0).all()
+    for i in range(2):
+        ref, _ = oracle.tau2d(m00[i], m01[i], m10[i], ti, sy.P, X[i], Y[i], w=0.5)
+        assert rel(tau[i], ref) < TOL
+
+
+def _golden_system(name):
+    from test_oracle import GOLDEN
+    return GOLDEN[name]
+
+
+@pytest.mark.parametrize("name", ["cfg0_spin1", "cfg0_spin2", "bdg18", "herm16"])
+@pytest.mark.parametrize("batch", [0, 5])
+def test_tau_bz_matches_reference_golden(gpu, name, batch):
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    sy = _golden_system(name)
+    E = complex(g["E"])
+    with gpu.Context.for_system(sy, batch=batch) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, g["k"], g["w"])
+    np.testing.assert_array_equal(r["iters"], g["iters"])
+    assert (r["status"] == 0).all()
+    assert rel(r["tau_bz"], g["tau_bz"]) < TOL
+    assert np.abs(r["trk"] - g["trk"]).max() < TOL * np.abs(g["trk"]).max()
+
+
+def test_config0_full_mesh_matches_oracle(gpu):
+    cfg = S.CONFIGS[0]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    ref = oracle.tau_bz(sy, E, k, w, nworkers=8)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, k, w)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL
+    assert np.abs(r["trk"] - ref["trk"]).max() < TOL * np.abs(ref["trk"]).max()
+
+
+def test_config1_sample_matches_oracle(gpu):
+    """Config 1 (n=128, 20 PLs, 25 R) on 8 k-points of its 64x64 mesh."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.array([0, 517, 1300, 2047, 2048, 3001, 3500, 4095])
+    ks, ws = k[idx], w[idx]
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=8)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, ks, ws)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL
+    assert np.abs(r["trk"] - ref["trk"]).max() < TOL * np.abs(ref["trk"]).max()
+
+
+def test_config1_full_mesh_properties(gpu):
+    """Full config-1 energy (4096 k): per-k results do not depend on how the k
+    are batched (bitwise), and the BZ sum is linear in the weights."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, k, w)
+    assert (r["status"] == 0).all()
+    with gpu.Context.for_system(sy, batch=97) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        sel = np.arange(0, 4096, 13)
+        r2 = ctx.tau_bz(E, k[sel], w[sel])
+    np.testing.assert_array_equal(r2["trk"], r["trk"][sel])
+    np.testing.assert_array_equal(r2["iters"], r["iters"][sel])
+    tr_bz = np.einsum("sll->s", r["tau_bz"])
+    assert rel(tr_bz, (w[:, None] * r["trk"]).sum(0)) < 1e-13
+    # DOS of each site is positive: -Im tr tau / pi > 0 (resolvent, Im E > 0)
+    assert (-tr_bz.imag > 0).all()
+
+
+def test_itermax_exceeded_is_reported(gpu):
+    sy = small_sys(lk=3, s=2, P=2, E=0.2 + 0.001j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(2)
+    with gpu.Context.for_system(sy, itermax=1) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, k, w, raise_on_error=False)
+    assert r["rc"] == 6
+    assert (r["status"] == 6).all()
+    ref = oracle.tau_bz(sy, E, k, w, itermax=1)
+    assert ref["rc"] == 3
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
