@@ -140,7 +140,8 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
     }
     __syncthreads();
   }
-  // D^T layout: lane l, register q -> C row (l % 16), C column 4*(l/16) + q
+  // v_mfma_f64_16x16x4 result layout: lane l, register q holds D[l/16 + 4q][l%16];
+  // D is the transposed block, so C row (l % 16), C column (l/16 + 4q)
   const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -149,7 +150,7 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         int gr = row0 + wm * 32 + i * 16 + l16;
-        int gc = col0 + wn * 32 + j * 16 + 4 * l4 + q;
+        int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
         if (gr < m && gc < n) {
           double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
           double2 r = cmul(alpha, v);

@@ -14,12 +14,14 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/skkr_gpu.h"
 #include "kernels_f64.h"
+#include "gjinv_blk.h"
 
 #define HIPCHK(x)                                                          \
   do {                                                                     \
@@ -92,6 +94,7 @@ struct skkr_ctx {
   size_t evused = 0;
   double dom_ms = 0.0, dom_flops = 0.0;
   long long dom_launches = 0;
+  int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
 };
 
 // ------------------------------------------------------------------------
@@ -252,6 +255,19 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
 
 static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
   if (nb <= 0) return SKKR_OK;
+  if (c->inv_kernel == 0) {  // blocked MFMA Gauss-Jordan
+    if (a.n <= 32)
+      gjinv_blk_f64_kernel<2, 2, 2><<<nb, 64, 0, c->stream>>>(a);
+    else if (a.n <= 64)
+      gjinv_blk_f64_kernel<4, 2, 2><<<nb, 256, 0, c->stream>>>(a);
+    else if (a.n <= 128)
+      gjinv_blk_f64_kernel<8, 2, 4><<<nb, 512, 0, c->stream>>>(a);
+    else
+      return SKKR_ERR_UNSUPPORTED;
+    HIPCHK(hipGetLastError());
+    c->launches++;
+    return SKKR_OK;
+  }
   if (a.n <= 32) {
     gjinv_f64_kernel<8, 8, 4, 4><<<nb, 64, 0, c->stream>>>(a);
   } else if (a.n <= 64) {
@@ -442,6 +458,7 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   HIPCHK(hipSetDevice(cfg->device));
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
+  if (const char* e = getenv("SKKR_INV_KERNEL")) c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : 0;
   c->n = cfg->lk * cfg->sites_per_pl;
   c->nn = c->n * c->n;
   c->ns = cfg->sites_per_pl;

@@ -1,0 +1,71 @@
+! ISO_C_BINDING interface to libskkr_gpu.so (include/skkr_gpu.h).
+! complex(c_double_complex) arrays are passed as the complex*16 arrays of the
+! reference (interleaved re,im; column-major).
+module skkr_gpu
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: SKKR_OK = 0, SKKR_ERR_ARG = 1, SKKR_ERR_HIP = 2, &
+       SKKR_ERR_NOMEM = 3, SKKR_ERR_SINGULAR = 4, SKKR_ERR_SGF_DIVERGED = 5, &
+       SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7
+  integer(c_int), parameter :: SKKR_FP64 = 0, SKKR_MIXED_FP32 = 1, SKKR_MIXED_BF16 = 2
+
+  type, bind(C) :: skkr_config
+     integer(c_int) :: lk, sites_per_pl, n_pl
+     real(c_double) :: errmax
+     integer(c_int) :: itermax_l, itermax_r, precision
+     real(c_double) :: fp64_dec_below_imE
+     integer(c_int) :: device, batch
+  end type skkr_config
+
+  type, bind(C) :: skkr_stats
+     real(c_double) :: ms_total, ms_fourier, ms_decimation, ms_sweeps, ms_accumulate, ms_refine
+     real(c_double) :: mean_iter_l, mean_iter_r
+     integer(c_int) :: max_iter, n_batches
+     integer(c_long_long) :: n_launches
+     integer(c_int) :: timing_detail
+     integer(c_long_long) :: dom_launches
+     real(c_double) :: dom_ms, dom_flops
+     character(kind=c_char) :: dom_name(32)
+  end type skkr_stats
+
+  interface
+     integer(c_int) function skkr_create(ctx, cfg) bind(C, name="skkr_create")
+       import :: c_int, c_ptr, skkr_config
+       type(c_ptr), intent(out) :: ctx
+       type(skkr_config), intent(in) :: cfg
+     end function skkr_create
+
+     subroutine skkr_destroy(ctx) bind(C, name="skkr_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end subroutine skkr_destroy
+
+     integer(c_int) function skkr_set_structure(ctx, n_R, R, gr_intra, gr_inter) &
+          bind(C, name="skkr_set_structure")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_R
+       real(c_double), intent(in) :: R(*)
+       complex(c_double_complex), intent(in) :: gr_intra(*), gr_inter(*)
+     end function skkr_set_structure
+
+     integer(c_int) function skkr_set_tinv(ctx, tinv_int, tinv_l, tinv_r) &
+          bind(C, name="skkr_set_tinv")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       complex(c_double_complex), intent(in) :: tinv_int(*), tinv_l(*), tinv_r(*)
+     end function skkr_set_tinv
+
+     integer(c_int) function skkr_tau_bz(ctx, e_re, e_im, nk, k, w, tau_bz, tr_tau_k, &
+          iters_k, status_k, stats) bind(C, name="skkr_tau_bz")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       real(c_double), value :: e_re, e_im
+       integer(c_int), value :: nk
+       real(c_double), intent(in) :: k(*), w(*)
+       complex(c_double_complex), intent(inout) :: tau_bz(*)
+       type(c_ptr), value :: tr_tau_k, iters_k, status_k, stats
+     end function skkr_tau_bz
+  end interface
+end module skkr_gpu

@@ -68,8 +68,9 @@ def uniform_kmesh(nside: int):
 
 def colmajor(a: np.ndarray) -> np.ndarray:
     """Stack of matrices [..., row, col] -> C-contiguous buffer whose memory is
-    column-major per matrix (Fortran ``complex*16 a(n,n)`` layout)."""
-    return np.ascontiguousarray(np.swapaxes(a, -1, -2))
+    column-major per matrix (Fortran ``complex*16 a(n,n)`` layout).  Always a
+    new array, so in-place device results never alias the caller's input."""
+    return np.array(np.swapaxes(a, -1, -2), order="C", copy=True)
 
 
 def from_colmajor(buf: np.ndarray) -> np.ndarray:
