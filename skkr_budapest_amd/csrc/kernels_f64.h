@@ -70,6 +70,25 @@ __global__ __launch_bounds__(256) void fourier_kernel(
   }
 }
 
+// Epilogues of the decimation products (surfgf.f:62-81), applied per element
+// of the 2n x n result [ay; by] * beta (DEC_BETA) or [ay; by] * alpha
+// (DEC_ALPHA):
+//   DEC_BETA : top rows  y3 = ay*beta : eps -= y3, x -= y3
+//              bottom    beta' = by*beta -> ABn bottom
+//   DEC_ALPHA: top rows  alpha' = ay*alpha -> ABn top, |alpha'|^2 partial
+//              bottom    y4 = by*alpha : eps -= y4
+// DEC_BETA runs first, so eps is updated as (eps - y3) - y4.
+enum { EPI_PLAIN = 0, EPI_DEC_BETA = 1, EPI_DEC_ALPHA = 2 };
+struct EpiArgs {
+  int mode;
+  int n;                       // chain block size (rows [0,n) are the top half)
+  double2* const* eps;
+  double2* const* x;
+  double2* const* abn;         // next [alpha; beta], 2n x n, ld 2n
+  double* norm;                // per (chain, work-group) partial |alpha'|^2
+  int tiles;                   // work-groups per chain
+};
+
 // ---------------------------------------------------------------------------
 // Batched complex GEMM on FP64 MFMA:
 //   C[b] = alpha * A[b](m x k) * B[b](k x n) + beta * C[b]
@@ -83,11 +102,15 @@ template <int TM, int TN, int KC>
 __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
     int m, int n, int k, const double2* const* __restrict__ Ap, int lda,
     const double2* const* __restrict__ Bp, int ldb, double2* const* __restrict__ Cp,
-    int ldc, double2 alpha, double2 beta, const int* __restrict__ active) {
+    int ldc, double2 alpha, double2 beta, const int* __restrict__ active, EpiArgs epi) {
   constexpr int NW = (TM / 32) * (TN / 32);
   constexpr int NT = NW * 64;
-  __shared__ double2 As[KC][TM];
-  __shared__ double2 Bs[TN][KC + 1];
+  // double-buffered LDS stages; the next stage is fetched into registers while
+  // the MFMAs of the current one run
+  __shared__ double2 As[2][KC][TM];
+  __shared__ double2 Bs[2][TN][KC + 1];
+  constexpr int LA = (TM * KC + NT - 1) / NT;  // A elements per thread per stage
+  constexpr int LB = (TN * KC + NT - 1) / NT;
   const int b = blockIdx.y;
   if (active && !active[b]) return;
   const int tiles_m = (m + TM - 1) / TM;
@@ -95,7 +118,7 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
   const int col0 = (blockIdx.x / tiles_m) * TN;
   const double2* __restrict__ A = Ap[b];
   const double2* __restrict__ B = Bp[b];
-  double2* __restrict__ C = Cp[b];
+  double2* __restrict__ C = Cp ? Cp[b] : nullptr;  // epilogue modes write elsewhere
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w % (TM / 32), wn = w / (TM / 32);
   dx4 acc_re[2][2], acc_im[2][2];
@@ -107,25 +130,51 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
       acc_im[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
     }
   const int l16 = lane & 15, l4 = lane >> 4;
+  double2 ra[LA], rb[LB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const int idx = tid + NT * u;
+      const int r = idx % TM, kk = idx / TM;
+      const int gr = row0 + r, gk = k0 + kk;
+      ra[u] = (idx < TM * KC && gr < m && gk < k) ? A[gr + (long long)gk * lda]
+                                                   : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int idx = tid + NT * u;
+      const int kk = idx % KC, c = idx / KC;
+      const int gk = k0 + kk, gc = col0 + c;
+      rb[u] = (idx < TN * KC && gk < k && gc < n) ? B[gk + (long long)gc * ldb]
+                                                   : make_double2(0.0, 0.0);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const int idx = tid + NT * u;
+      if (idx < TM * KC) As[buf][idx / TM][idx % TM] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int idx = tid + NT * u;
+      if (idx < TN * KC) Bs[buf][idx / KC][idx % KC] = rb[u];
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
   for (int k0 = 0; k0 < k; k0 += KC) {
-    for (int idx = tid; idx < TM * KC; idx += NT) {
-      int r = idx % TM, kk = idx / TM;
-      int gr = row0 + r, gk = k0 + kk;
-      As[kk][r] = (gr < m && gk < k) ? A[gr + (long long)gk * lda] : make_double2(0.0, 0.0);
-    }
-    for (int idx = tid; idx < TN * KC; idx += NT) {
-      int kk = idx % KC, c = idx / KC;
-      int gk = k0 + kk, gc = col0 + c;
-      Bs[c][kk] = (gk < k && gc < n) ? B[gk + (long long)gc * ldb] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
+    const bool more = (k0 + KC < k);
+    if (more) fetch(k0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
       double2 af[2], bf[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = As[ks + l4][wm * 32 + i * 16 + l16];
+      for (int i = 0; i < 2; ++i) af[i] = As[buf][ks + l4][wm * 32 + i * 16 + l16];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Bs[wn * 32 + j * 16 + l16][ks + l4];
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][wn * 32 + j * 16 + l16][ks + l4];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -138,27 +187,76 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
           acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].y, af[i].x, acc_im[i][j], 0, 0, 0);
         }
     }
+    if (more) stash(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
   // v_mfma_f64_16x16x4 result layout: lane l, register q holds D[l/16 + 4q][l%16];
   // D is the transposed block, so C row (l % 16), C column (l/16 + 4q)
-  const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+  if (epi.mode == EPI_PLAIN) {
+    const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int gr = row0 + wm * 32 + i * 16 + l16;
+          int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
+          if (gr < m && gc < n) {
+            double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
+            double2 r = cmul(alpha, v);
+            long long o = gr + (long long)gc * ldc;
+            if (use_beta) r = cadd(r, cmul(beta, C[o]));
+            C[o] = r;
+          }
+        }
+    return;
+  }
+  const int hn = epi.n;
+  double2* __restrict__ eps = epi.eps[b];
+  double2* __restrict__ xx = epi.x[b];
+  double2* __restrict__ abn = epi.abn[b];
+  double part = 0.0;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int gr = row0 + wm * 32 + i * 16 + l16;
-        int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
+        const int gr = row0 + wm * 32 + i * 16 + l16;
+        const int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
         if (gr < m && gc < n) {
-          double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
-          double2 r = cmul(alpha, v);
-          long long o = gr + (long long)gc * ldc;
-          if (use_beta) r = cadd(r, cmul(beta, C[o]));
-          C[o] = r;
+          const double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
+          if (gr < hn) {
+            const long long o = gr + (long long)gc * hn;
+            if (epi.mode == EPI_DEC_BETA) {
+              eps[o] = csub(eps[o], v);
+              xx[o] = csub(xx[o], v);
+            } else {
+              abn[gr + (long long)gc * 2 * hn] = v;
+              part += v.x * v.x + v.y * v.y;
+            }
+          } else {
+            const long long o = (gr - hn) + (long long)gc * hn;
+            if (epi.mode == EPI_DEC_BETA)
+              abn[gr + (long long)gc * 2 * hn] = v;
+            else
+              eps[o] = csub(eps[o], v);
+          }
         }
       }
+  if (epi.mode == EPI_DEC_ALPHA) {
+    // deterministic work-group sum of |alpha'|^2 (tree in LDS, fixed order)
+    __shared__ double red[NT];
+    red[tid] = part;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (tid < st) red[tid] += red[tid + st];
+      __syncthreads();
+    }
+    if (tid == 0) epi.norm[(long long)b * epi.tiles + blockIdx.x] = red[0];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -378,52 +476,50 @@ __global__ __launch_bounds__(256) void dec_init_kernel(
   }
 }
 
-// After C1 = [ay;by]*alpha, C2 = [ay;by]*beta (2n x n, ld 2n):
-//   alpha' = ay*alpha (C1 top), beta' = by*beta (C2 bottom),
-//   eps <- eps - ay*beta - by*alpha, x <- x - ay*beta, err = |alpha'|_F.
-// Convergence bookkeeping of surfgf.f:76-103 per chain (chains b < nhalf are
-// left leads with itermax_l, the others right leads with itermax_r).
-__global__ __launch_bounds__(256) void dec_update_kernel(
-    int n, int iter, int itermax_l, int itermax_r, int nhalf, double errmax,
-    const double2* const* C1,
-    const double2* const* C2, double2* const* AB, double2* const* eps,
-    double2* const* x, int* active, int* iters, int* status, int* n_active) {
-  __shared__ double red[256];
+// Convergence bookkeeping of surfgf.f:76-103 per chain after an iteration:
+// err = |alpha'|_F from the work-group partials (summed in order); chains
+// b < nhalf are left leads (itermax_l), the others right leads (itermax_r).
+__global__ __launch_bounds__(256) void dec_check_kernel(
+    int nchains, int tiles, int iter, int itermax_l, int itermax_r, int nhalf, double errmax,
+    const double* __restrict__ norm, int* active, int* iters, int* status, int* n_active) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nchains || !active[b]) return;
+  double s = 0.0;
+  for (int t = 0; t < tiles; ++t) s += norm[(long long)b * tiles + t];
+  const double err = sqrt(s);
+  const int itermax = (b < nhalf) ? itermax_l : itermax_r;
+  if (err < errmax || iter > itermax) {
+    active[b] = 0;
+    iters[b] = iter;
+    if (iter > itermax) status[b] |= 4;
+  } else if (err > 1.0e10) {
+    active[b] = 0;
+    iters[b] = iter;
+    status[b] |= 2;
+  } else {
+    atomicAdd(n_active, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Site-diagonal lk x lk blocks of an n x n matrix and their traces (the part
+// of the PL-diagonal tau that the BZ sum keeps, tau2d.f:127-141).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void extract_sites_kernel(
+    int n, int lk, const double2* const* G, double2* const* tau, double2* const* tr) {
   const int b = blockIdx.x;
-  if (!active[b]) return;
-  double sum = 0.0;
-  const double2* __restrict__ c1 = C1[b];
-  const double2* __restrict__ c2 = C2[b];
-  for (int e = threadIdx.x; e < n * n; e += 256) {
-    int r = e % n, c = e / n;
-    long long top = r + (long long)c * 2 * n, bot = top + n;
-    double2 an = c1[top], bn = c2[bot], y3 = c2[top], y4 = c1[bot];
-    AB[b][top] = an;
-    AB[b][bot] = bn;
-    eps[b][e] = csub(csub(eps[b][e], y3), y4);
-    x[b][e] = csub(x[b][e], y3);
-    sum += an.x * an.x + an.y * an.y;
+  const double2* __restrict__ g = G[b];
+  double2* __restrict__ t = tau[b];
+  const int ns = n / lk;
+  for (int e = threadIdx.x; e < ns * lk * lk; e += 256) {
+    const int st = e / (lk * lk), rem = e % (lk * lk);
+    const int l = rem % lk, lp = rem / lk;
+    t[e] = g[(st * lk + l) + (long long)(st * lk + lp) * n];
   }
-  red[threadIdx.x] = sum;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double err = sqrt(red[0]);
-    const int itermax = (b < nhalf) ? itermax_l : itermax_r;
-    if (err < errmax || iter > itermax) {
-      active[b] = 0;
-      iters[b] = iter;
-      if (iter > itermax) status[b] |= 4;
-    } else if (err > 1.0e10) {
-      active[b] = 0;
-      iters[b] = iter;
-      status[b] |= 2;
-    } else {
-      atomicAdd(n_active, 1);
-    }
+  for (int st = threadIdx.x; st < ns; st += 256) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int l = 0; l < lk; ++l) s = cadd(s, g[(st * lk + l) + (long long)(st * lk + l) * n]);
+    tr[b][st] = s;
   }
 }
 

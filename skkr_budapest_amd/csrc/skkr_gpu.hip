@@ -74,18 +74,18 @@ struct skkr_ctx {
   double* dk = nullptr;
   double* dw = nullptr;
   z *M00 = nullptr, *M01 = nullptr, *M10 = nullptr;
-  z *eps = nullptr, *xs = nullptr, *AB = nullptr, *Yinv = nullptr, *AYBY = nullptr,
-    *C1 = nullptr, *C2 = nullptr;
+  z *eps = nullptr, *xs = nullptr, *AB = nullptr, *Yinv = nullptr, *AYBY = nullptr;
+  double* dnorm = nullptr;  // |alpha'|^2 partials per (chain, work-group)
   int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
       *nactive = nullptr, *status_sw = nullptr;
-  z *Ys = nullptr, *W = nullptr, *Zb = nullptr, *Xc = nullptr, *tauk = nullptr,
-    *trk = nullptr;
+  z *Gs = nullptr, *Zs = nullptr, *Ycur = nullptr, *W = nullptr, *Zb = nullptr,
+    *Xc = nullptr, *tauk = nullptr, *trk = nullptr;
   z* taubz = nullptr;
   // pointer arrays
-  PtrArr p_ml, p_mr, p_m0, p_tl, p_AB, p_eps, p_x, p_Yinv, p_AYBY, p_C1, p_C2,
-      p_alpha, p_beta, p_sigma;
-  PtrArr p_M00, p_M01, p_M10, p_W, p_Z, p_X;
-  std::vector<PtrArr> p_Ys, p_tau, p_tr;
+  PtrArr p_ml, p_mr, p_m0, p_tl, p_eps, p_x, p_Yinv, p_AYBY, p_sigma;
+  PtrArr p_AB[2], p_alpha[2], p_beta[2];  // ping-pong [alpha; beta]
+  PtrArr p_M00, p_M01, p_M10, p_W, p_Z, p_X, p_Y;
+  std::vector<PtrArr> p_G, p_Zs, p_tau, p_tr;
   // stats
   long long launches = 0;
   int timing_detail = 0;
@@ -95,6 +95,7 @@ struct skkr_ctx {
   double dom_ms = 0.0, dom_flops = 0.0;
   long long dom_launches = 0;
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
+  int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 128
 };
 
 // ------------------------------------------------------------------------
@@ -136,18 +137,19 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_t(c, &c->M10, B * nn));
   CHK(dalloc_t(c, &c->eps, B2 * nn));
   CHK(dalloc_t(c, &c->xs, B2 * nn));
-  CHK(dalloc_t(c, &c->AB, B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->AB, 2 * B2 * 2 * nn));
+  CHK(dalloc_t(c, &c->dnorm, B2 * 64));
   CHK(dalloc_t(c, &c->Yinv, B2 * nn));
   CHK(dalloc_t(c, &c->AYBY, B2 * 2 * nn));
-  CHK(dalloc_t(c, &c->C1, B2 * 2 * nn));
-  CHK(dalloc_t(c, &c->C2, B2 * 2 * nn));
   CHK(dalloc_t(c, &c->active, B2));
   CHK(dalloc_t(c, &c->iters, B2));
   CHK(dalloc_t(c, &c->status, B2));
   CHK(dalloc_t(c, &c->itermax, B2));
   CHK(dalloc_t(c, &c->nactive, 1));
   CHK(dalloc_t(c, &c->status_sw, (size_t)B));
-  CHK(dalloc_t(c, &c->Ys, P * B * nn));
+  CHK(dalloc_t(c, &c->Gs, P * B * nn));
+  CHK(dalloc_t(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
+  CHK(dalloc_t(c, &c->Ycur, B * nn));
   CHK(dalloc_t(c, &c->W, B * nn));
   CHK(dalloc_t(c, &c->Zb, B * nn));
   CHK(dalloc_t(c, &c->Xc, B * nn));
@@ -166,32 +168,35 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(fill(&c->p_mr, [&](size_t j) { return j < Bs ? c->M01 + j * nn : c->M10 + (j - Bs) * nn; }, B2));
   CHK(fill(&c->p_m0, [&](size_t j) { return c->M00 + (j % Bs) * nn; }, B2));
   CHK(fill(&c->p_tl, [&](size_t j) { return j < Bs ? c->dTl : c->dTr; }, B2));
-  CHK(fill(&c->p_AB, [&](size_t j) { return c->AB + j * 2 * nn; }, B2));
-  CHK(fill(&c->p_alpha, [&](size_t j) { return c->AB + j * 2 * nn; }, B2));
-  CHK(fill(&c->p_beta, [&](size_t j) { return c->AB + j * 2 * nn + n; }, B2));
+  for (int h = 0; h < 2; ++h) {
+    z* base = c->AB + (size_t)h * B2 * 2 * nn;
+    CHK(fill(&c->p_AB[h], [&](size_t j) { return base + j * 2 * nn; }, B2));
+    CHK(fill(&c->p_alpha[h], [&](size_t j) { return base + j * 2 * nn; }, B2));
+    CHK(fill(&c->p_beta[h], [&](size_t j) { return base + j * 2 * nn + n; }, B2));
+  }
   CHK(fill(&c->p_eps, [&](size_t j) { return c->eps + j * nn; }, B2));
   CHK(fill(&c->p_x, [&](size_t j) { return c->xs + j * nn; }, B2));
   CHK(fill(&c->p_Yinv, [&](size_t j) { return c->Yinv + j * nn; }, B2));
   CHK(fill(&c->p_AYBY, [&](size_t j) { return c->AYBY + j * 2 * nn; }, B2));
-  CHK(fill(&c->p_C1, [&](size_t j) { return c->C1 + j * 2 * nn; }, B2));
-  CHK(fill(&c->p_C2, [&](size_t j) { return c->C2 + j * 2 * nn; }, B2));
-  // self-energy outputs: Y_1 into Ys[0], X_P into Xc
-  CHK(fill(&c->p_sigma, [&](size_t j) { return j < Bs ? c->Ys + j * nn : c->Xc + (j - Bs) * nn; }, B2));
+  // self-energy outputs: Y_1 into Ycur, X_P into Xc
+  CHK(fill(&c->p_sigma, [&](size_t j) { return j < Bs ? c->Ycur + j * nn : c->Xc + (j - Bs) * nn; }, B2));
   CHK(fill(&c->p_M00, [&](size_t j) { return c->M00 + j * nn; }, Bs));
   CHK(fill(&c->p_M01, [&](size_t j) { return c->M01 + j * nn; }, Bs));
   CHK(fill(&c->p_M10, [&](size_t j) { return c->M10 + j * nn; }, Bs));
   CHK(fill(&c->p_W, [&](size_t j) { return c->W + j * nn; }, Bs));
   CHK(fill(&c->p_Z, [&](size_t j) { return c->Zb + j * nn; }, Bs));
   CHK(fill(&c->p_X, [&](size_t j) { return c->Xc + j * nn; }, Bs));
-  c->p_Ys.assign(P, PtrArr());
+  CHK(fill(&c->p_Y, [&](size_t j) { return c->Ycur + j * nn; }, Bs));
+  c->p_G.assign(P, PtrArr());
+  c->p_Zs.assign(P, PtrArr());
   c->p_tau.assign(P, PtrArr());
   c->p_tr.assign(P, PtrArr());
   for (size_t p = 0; p < P; ++p) {
-    CHK(fill(&c->p_Ys[p], [&](size_t j) { return c->Ys + (p * Bs + j) * nn; }, Bs));
+    CHK(fill(&c->p_G[p], [&](size_t j) { return c->Gs + (p * Bs + j) * nn; }, Bs));
+    if (p + 1 < P) CHK(fill(&c->p_Zs[p], [&](size_t j) { return c->Zs + (p * Bs + j) * nn; }, Bs));
     CHK(fill(&c->p_tau[p], [&](size_t j) { return c->tauk + (j * P * ns + p * ns) * lk * lk; }, Bs));
     CHK(fill(&c->p_tr[p], [&](size_t j) { return c->trk + j * P * ns + p * ns; }, Bs));
   }
-  // sweep Y_1 lives in Ys[0]; p_sigma above points there
   c->B = B;
   return SKKR_OK;
 }
@@ -229,24 +234,35 @@ static void dom_collect(skkr_ctx* c) {
   c->evflops.clear();
 }
 
-static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
-                        int ldb, z** C, int ldc, z alpha, z beta, int nb,
-                        const int* active) {
+// Work-group tile of a GEMM shape: the context's tuned choice (skkr_autotune
+// or SKKR_GEMM_TILE), else the largest tile that the shape fills.
+static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
+  int t = (m >= 128 && n >= 128) ? 128 : ((m > 32 || n > 32) ? 64 : 32);
+  if (c && c->gemm_tile_force > 0) t = c->gemm_tile_force;
+  *tm = t;
+  *tiles = ((m + t - 1) / t) * ((n + t - 1) / t);
+}
+
+static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
+                        z** C, int ldc, z alpha, z beta, int nb, const int* active,
+                        const EpiArgs* epi_in = nullptr) {
   if (nb <= 0) return SKKR_OK;
+  EpiArgs epi;
+  memset(&epi, 0, sizeof(epi));
+  if (epi_in) epi = *epi_in;
+  int tm, tiles;
+  gemm_tile(c, m, n, &tm, &tiles);
+  dim3 grid(tiles, nb);
   dom_begin(c);
-  if (m >= 128 && n >= 128) {
-    dim3 grid(((m + 127) / 128) * ((n + 127) / 128), nb);
+  if (tm == 128)
     zgemm_f64_kernel<128, 128, 16><<<grid, 1024, 0, c->stream>>>(
-        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
-  } else if (m > 32 || n > 32) {
-    dim3 grid(((m + 63) / 64) * ((n + 63) / 64), nb);
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (tm == 64)
     zgemm_f64_kernel<64, 64, 16><<<grid, 256, 0, c->stream>>>(
-        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
-  } else {
-    dim3 grid(((m + 31) / 32) * ((n + 31) / 32), nb);
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else
     zgemm_f64_kernel<32, 32, 16><<<grid, 64, 0, c->stream>>>(
-        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active);
-  }
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   HIPCHK(hipGetLastError());
   dom_end(c, 8.0 * m * n * (double)k * nb);
   c->launches++;
@@ -313,19 +329,25 @@ static int run_fourier(skkr_ctx* c, int nk) {
 }
 
 // Decimation of chains j in [0, 2B) whose active flag the caller set; chains
-// use p_ml/p_mr/p_m0/p_tl.  Leaves g_s = x^-1 in Yinv and the self-energy
-// ml g_s mr in p_sigma when want_sigma.
+// use p_ml/p_mr/p_m0/p_tl.  Per iteration (surfgf.f:47-96): y = eps^-1,
+// [ay; by] = [alpha; beta] y, then [ay; by] beta and [ay; by] alpha with the
+// updates of eps, x and the next [alpha; beta] fused into the GEMM epilogues.
+// Leaves g_s = x^-1 in Yinv and the self-energy ml g_s mr in p_sigma when
+// want_sigma.
 static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
   const int n = c->n, B2 = 2 * c->B;
   dec_init_kernel<<<B2, 256, 0, c->stream>>>(n, c->cfg.lk, c->p_ml.d, c->p_mr.d, c->p_m0.d,
-                                             add_tinv ? c->p_tl.d : nullptr, c->p_AB.d,
+                                             add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
                                              c->p_eps.d, c->p_x.d);
   HIPCHK(hipGetLastError());
-  // chains never active are skipped by the kernels; keep a copy of the mask
   std::vector<int> act(B2);
   HIPCHK(hipMemcpyAsync(act.data(), c->active, B2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   const int itermax = std::max(c->cfg.itermax_l, c->cfg.itermax_r);
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  int tm, tiles;
+  gemm_tile(c, 2 * n, n, &tm, &tiles);
+  int cur = 0;
   for (int iter = 1; iter <= itermax + 1; ++iter) {
     InvArgs ia = inv_args(c);
     ia.base = c->p_eps.d;
@@ -333,24 +355,33 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
     ia.active = c->active;
     ia.status = c->status;
     CHK(launch_gjinv(c, ia, B2));
-    const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB.d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
                      one, zero, B2, c->active));
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha.d, 2 * n, c->p_C1.d, 2 * n,
-                     one, zero, B2, c->active));
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta.d, 2 * n, c->p_C2.d, 2 * n,
-                     one, zero, B2, c->active));
+    EpiArgs e;
+    memset(&e, 0, sizeof(e));
+    e.n = n;
+    e.eps = c->p_eps.d;
+    e.x = c->p_x.d;
+    e.abn = c->p_AB[cur ^ 1].d;
+    e.norm = c->dnorm;
+    e.tiles = tiles;
+    e.mode = EPI_DEC_BETA;
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
+                     one, zero, B2, c->active, &e));
+    e.mode = EPI_DEC_ALPHA;
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
+                     one, zero, B2, c->active, &e));
     HIPCHK(hipMemsetAsync(c->nactive, 0, sizeof(int), c->stream));
-    dec_update_kernel<<<B2, 256, 0, c->stream>>>(n, iter, c->cfg.itermax_l, c->cfg.itermax_r,
-                                                 c->B, c->cfg.errmax, c->p_C1.d,
-                                                 c->p_C2.d, c->p_AB.d, c->p_eps.d, c->p_x.d,
-                                                 c->active, c->iters, c->status, c->nactive);
+    dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
+        B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
+        c->active, c->iters, c->status, c->nactive);
     HIPCHK(hipGetLastError());
-    c->launches += 2;
+    c->launches++;
     int nact = 0;
     HIPCHK(hipMemcpyAsync(&nact, c->nactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (nact == 0) break;
+    cur ^= 1;
   }
   // g_s = x^-1 for every chain that was started
   HIPCHK(hipMemcpyAsync(c->active, act.data(), B2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -361,7 +392,6 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
   ia.status = c->status;
   CHK(launch_gjinv(c, ia, B2));
   if (want_sigma) {
-    const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
     // T = ml g_s (into AYBY as n x n), sigma = T mr
     CHK(launch_zgemm(c, n, n, n, c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero, B2,
                      c->active));
@@ -371,50 +401,57 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
   return SKKR_OK;
 }
 
-// Godfrin sweeps for nb k-points: Y_1 in Ys[0], X_P in Xc, blocks in M00/M01/M10.
+// PL-diagonal blocks of tau for nb k-points: Y_1 in Ycur, X_P in Xc, blocks
+// in M00/M01/M10.  Recursive Green's-function form of the block-tridiagonal
+// inverse (same tau_pp as the Godfrin sweeps of tau2d.f:53-149):
+//   forward  g_p = (T_p - Y_p)^-1, Z_p = g_p M01, Y_{p+1} = M10 Z_p;
+//   last     G_P = (T_P - X_P - Y_P)^-1;
+//   backward G_p = g_p + Z_p (G_{p+1} (M10 g_p)).
+// One inversion per PL instead of three; the rest are GEMMs.
+static int launch_extract(skkr_ctx* c, int p, int nb) {
+  extract_sites_kernel<<<nb, 256, 0, c->stream>>>(c->n, c->cfg.lk, c->p_G[p].d, c->p_tau[p].d,
+                                                  c->p_tr[p].d);
+  HIPCHK(hipGetLastError());
+  c->launches++;
+  return SKKR_OK;
+}
+
 static int run_sweeps(skkr_ctx* c, int nb) {
   const int n = c->n, P = c->cfg.n_pl, s = c->cfg.sites_per_pl, lk = c->cfg.lk;
   const size_t tblk = (size_t)s * lk * lk;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  // Y sweep (tau2d.f:53-85)
   for (int p = 0; p < P - 1; ++p) {
     InvArgs ia = inv_args(c);
     ia.base = c->p_M00.d;
     ia.tinv = c->dTint + p * tblk;
-    ia.sub1 = c->p_Ys[p].d;
-    ia.out = c->p_W.d;
+    ia.sub1 = c->p_Y.d;
+    ia.out = c->p_G[p].d;
     ia.status = c->status_sw;
     CHK(launch_gjinv(c, ia, nb));
-    CHK(launch_zgemm(c, n, n, n, c->p_W.d, n, c->p_M01.d, n, c->p_Z.d, n, one, zero, nb, nullptr));
-    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_Z.d, n, c->p_Ys[p + 1].d, n, one, zero, nb,
+    CHK(launch_zgemm(c, n, n, n, c->p_G[p].d, n, c->p_M01.d, n, c->p_Zs[p].d, n, one, zero, nb,
+                     nullptr));
+    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_Zs[p].d, n, c->p_Y.d, n, one, zero, nb,
                      nullptr));
   }
-  // X sweep (tau2d.f:89-109) with the diagonal blocks (:114-149) fused
-  for (int p = P - 1; p >= 0; --p) {
-    if (p < P - 1)
-      CHK(launch_zgemm(c, n, n, n, c->p_M01.d, n, c->p_Z.d, n, c->p_X.d, n, one, zero, nb,
-                       nullptr));
+  {
     InvArgs ia = inv_args(c);
     ia.base = c->p_M00.d;
-    ia.tinv = c->dTint + p * tblk;
+    ia.tinv = c->dTint + (P - 1) * tblk;
     ia.sub1 = c->p_X.d;
-    ia.sub2 = c->p_Ys[p].d;
-    ia.tau = c->p_tau[p].d;
-    ia.tr = c->p_tr[p].d;
+    ia.sub2 = c->p_Y.d;
+    ia.out = c->p_G[P - 1].d;
     ia.status = c->status_sw;
-    ia.mode = 1;
     CHK(launch_gjinv(c, ia, nb));
-    if (p > 0) {
-      InvArgs ib = inv_args(c);
-      ib.base = c->p_M00.d;
-      ib.tinv = c->dTint + p * tblk;
-      ib.sub1 = c->p_X.d;
-      ib.out = c->p_W.d;
-      ib.status = c->status_sw;
-      CHK(launch_gjinv(c, ib, nb));
-      CHK(launch_zgemm(c, n, n, n, c->p_W.d, n, c->p_M10.d, n, c->p_Z.d, n, one, zero, nb,
-                       nullptr));
-    }
+    CHK(launch_extract(c, P - 1, nb));
+  }
+  for (int p = P - 2; p >= 0; --p) {
+    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_G[p].d, n, c->p_W.d, n, one, zero, nb,
+                     nullptr));
+    CHK(launch_zgemm(c, n, n, n, c->p_G[p + 1].d, n, c->p_W.d, n, c->p_Z.d, n, one, zero, nb,
+                     nullptr));
+    CHK(launch_zgemm(c, n, n, n, c->p_Zs[p].d, n, c->p_Z.d, n, c->p_G[p].d, n, one, one, nb,
+                     nullptr));
+    CHK(launch_extract(c, p, nb));
   }
   return SKKR_OK;
 }
@@ -459,6 +496,10 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
   if (const char* e = getenv("SKKR_INV_KERNEL")) c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : 0;
+  if (const char* e = getenv("SKKR_GEMM_TILE")) {
+    const int t = atoi(e);
+    if (t == 32 || t == 64 || t == 128) c->gemm_tile_force = t;
+  }
   c->n = cfg->lk * cfg->sites_per_pl;
   c->nn = c->n * c->n;
   c->ns = cfg->sites_per_pl;
@@ -539,8 +580,8 @@ int skkr_set_tinv(skkr_ctx* c, const double* ti, const double* tl, const double*
 static int auto_batch(skkr_ctx* c, int nk) {
   int B = c->cfg.batch;
   if (B <= 0) {
-    // ~ 22 n^2 + P n^2 complex per k; keep a batch under ~24 GB
-    double per_k = (22.0 + c->cfg.n_pl) * c->nn * 16.0;
+    // ~ 28 n^2 + 2P n^2 complex per k; keep a batch under ~24 GB
+    double per_k = (28.0 + 2.0 * c->cfg.n_pl) * c->nn * 16.0;
     B = (int)std::min(1024.0, std::max(1.0, 24e9 / per_k));
   }
   return std::max(1, std::min(B, nk));
@@ -759,7 +800,7 @@ static int dec_host(skkr_ctx* c, int nb, const double* ml, const double* m0, con
   if (x_out)
     HIPCHK(hipMemcpyAsync(x_out, c->Yinv + off * nn, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   if (sigma_out) {
-    z* src = (side == 'R') ? c->Xc : c->Ys;
+    z* src = (side == 'R') ? c->Xc : c->Ycur;
     HIPCHK(hipMemcpyAsync(sigma_out, src, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   }
   std::vector<int> hi(2 * B), hs(2 * B);
@@ -806,7 +847,7 @@ int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
   HIPCHK(hipMemcpyAsync(c->M01, m01, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->M10, m10, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->Xc, x, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->Ys, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Ycur, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
   CHK(run_sweeps(c, nb));
   long long ne = (long long)tsz * nb;
