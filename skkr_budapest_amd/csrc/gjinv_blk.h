@@ -63,7 +63,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
           const int c = (TJ * wc + j) * 16 + l16;
           double2 v;
           if (r < n && c < n) {
-            const long long o = r + (long long)c * n;
+            const long long o = r + (long long)c * (args.ld ? args.ld : n);
             v = base[o];
             if (args.tinv && (r / lk) == (c / lk))
               v = cadd(args.tinv[(long long)(r / lk) * lk * lk + (r % lk) + (c % lk) * lk], v);
@@ -266,7 +266,8 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
         for (int q = 0; q < 4; ++q) {
           const int r = (TI * wr + i) * 16 + l4 + 4 * q;
           const int c = (TJ * wc + j) * 16 + l16;
-          if (r < n && c < n) out[r + (long long)c * n] = make_double2(mre[i][j][q], mim[i][j][q]);
+          if (r < n && c < n)
+            out[r + (long long)c * (args.ldo ? args.ldo : n)] = make_double2(mre[i][j][q], mim[i][j][q]);
         }
   } else {
     double2* __restrict__ tau = args.tau[b];

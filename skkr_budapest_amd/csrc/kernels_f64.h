@@ -274,6 +274,7 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
 // ---------------------------------------------------------------------------
 struct InvArgs {
   int n, lk;
+  int ld, ldo;                // leading dimensions of the inputs / of out (0 = n)
   const double2* const* base; // per batch (may repeat)
   const double2* tinv;        // site blocks lk*lk, shared by the batch (or null)
   const double2* const* sub1; // or null
@@ -315,7 +316,8 @@ __global__ __launch_bounds__(TR * TC) void gjinv_f64_kernel(InvArgs args) {
         int r = tr + TR * i, c = tc + TC * j;
         double2 v = make_double2(0.0, 0.0);
         if (r < n && c < n) {
-          long long o = r + (long long)c * n;
+          const int ld = args.ld ? args.ld : n;
+          long long o = r + (long long)c * ld;
           v = base[o];
           if (args.tinv && (r / lk) == (c / lk)) {
             int st = r / lk;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(TR * TC) void gjinv_f64_kernel(InvArgs args) {
 #pragma unroll
       for (int j = 0; j < CPT; ++j) {
         int rho = tr + TR * i, gam = tc + TC * j;
-        if (rho < n && gam < n) out[pinv[rho] + (long long)perm[gam] * n] = a[i][j];
+        if (rho < n && gam < n) out[pinv[rho] + (long long)perm[gam] * (args.ldo ? args.ldo : n)] = a[i][j];
       }
   } else {
     const int lk = args.lk;
@@ -499,6 +501,30 @@ __global__ __launch_bounds__(256) void dec_check_kernel(
   } else {
     atomicAdd(n_active, 1);
   }
+}
+
+// Assembly a = base + blockdiag(tinv) - sub1 - sub2 into a dense n x n buffer
+// (the load stage of the inversion kernels, for sizes inverted by blocks).
+__global__ __launch_bounds__(256) void assemble_kernel(InvArgs args, double2* const* dst) {
+  const int b = blockIdx.y;
+  if (args.active && !args.active[b]) return;
+  const int n = args.n, lk = args.lk, ld = args.ld ? args.ld : n;
+  const long long e = blockIdx.x * 256ll + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const long long o = r + (long long)c * ld;
+  double2 v = args.base[b][o];
+  if (args.tinv && r / lk == c / lk)
+    v = cadd(args.tinv[(long long)(r / lk) * lk * lk + (r % lk) + (c % lk) * lk], v);
+  if (args.sub1) v = csub(v, args.sub1[b][o]);
+  if (args.sub2) v = csub(v, args.sub2[b][o]);
+  dst[b][e] = v;
+}
+
+// out[b] = in[b] + off (device-side views of sub-blocks)
+__global__ void ptr_offset_kernel(int nb, double2* const* in, long long off, double2** out) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b < nb) out[b] = in[b] + off;
 }
 
 // ---------------------------------------------------------------------------

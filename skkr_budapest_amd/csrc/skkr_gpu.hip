@@ -76,6 +76,11 @@ struct skkr_ctx {
   z *M00 = nullptr, *M01 = nullptr, *M10 = nullptr;
   z *eps = nullptr, *xs = nullptr, *AB = nullptr, *Yinv = nullptr, *AYBY = nullptr;
   double* dnorm = nullptr;  // |alpha'|^2 partials per (chain, work-group)
+  // 2x2 block inversion scratch (n > 128): assembled matrix, C = A21 A11^-1,
+  // D = A11^-1 A12, and views of sub-blocks
+  z *Asc = nullptr, *Csc = nullptr, *Dsc = nullptr;
+  PtrArr p_Asc, p_A21, p_A12, p_A22, p_Csc, p_Dsc;
+  z** dv_out[4] = {nullptr, nullptr, nullptr, nullptr};  // out + offsets (B11, B21, B12, B22)
   int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
       *nactive = nullptr, *status_sw = nullptr;
   z *Gs = nullptr, *Zs = nullptr, *Ycur = nullptr, *W = nullptr, *Zb = nullptr,
@@ -155,6 +160,13 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_t(c, &c->Xc, B * nn));
   CHK(dalloc_t(c, &c->tauk, (size_t)B * P * ns * lk * lk / (ns ? 1 : 1)));
   CHK(dalloc_t(c, &c->trk, (size_t)B * P * ns));
+  if (n > 128) {
+    const size_t h1 = 128, h2 = n - 128;
+    CHK(dalloc_t(c, &c->Asc, B2 * nn));
+    CHK(dalloc_t(c, &c->Csc, B2 * h1 * h2));
+    CHK(dalloc_t(c, &c->Dsc, B2 * h1 * h2));
+    for (int v = 0; v < 4; ++v) CHK(dalloc_t(c, &c->dv_out[v], B2));
+  }
   std::vector<z*> v(B2);
   auto fill = [&](PtrArr* pa, auto fn, size_t cnt) -> int {
     std::vector<z*> t(cnt);
@@ -180,6 +192,15 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(fill(&c->p_AYBY, [&](size_t j) { return c->AYBY + j * 2 * nn; }, B2));
   // self-energy outputs: Y_1 into Ycur, X_P into Xc
   CHK(fill(&c->p_sigma, [&](size_t j) { return j < Bs ? c->Ycur + j * nn : c->Xc + (j - Bs) * nn; }, B2));
+  if (n > 128) {
+    const size_t h1 = 128, h2 = n - 128;
+    CHK(fill(&c->p_Asc, [&](size_t j) { return c->Asc + j * nn; }, B2));
+    CHK(fill(&c->p_A21, [&](size_t j) { return c->Asc + j * nn + h1; }, B2));
+    CHK(fill(&c->p_A12, [&](size_t j) { return c->Asc + j * nn + h1 * n; }, B2));
+    CHK(fill(&c->p_A22, [&](size_t j) { return c->Asc + j * nn + h1 + h1 * n; }, B2));
+    CHK(fill(&c->p_Csc, [&](size_t j) { return c->Csc + j * h1 * h2; }, B2));
+    CHK(fill(&c->p_Dsc, [&](size_t j) { return c->Dsc + j * h1 * h2; }, B2));
+  }
   CHK(fill(&c->p_M00, [&](size_t j) { return c->M00 + j * nn; }, Bs));
   CHK(fill(&c->p_M01, [&](size_t j) { return c->M01 + j * nn; }, Bs));
   CHK(fill(&c->p_M10, [&](size_t j) { return c->M10 + j * nn; }, Bs));
@@ -298,6 +319,58 @@ static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
   return SKKR_OK;
 }
 
+// Inverse for any supported n: the register-resident kernel for n <= 128;
+// for 128 < n <= 256 a 2x2 block elimination around A11 (128 x 128):
+//   B11 = A11^-1 + D S^-1 C, B12 = -D S^-1, B21 = -S^-1 C, B22 = S^-1,
+//   C = A21 A11^-1, D = A11^-1 A12, S = A22 - C A12.
+// No rows move between the halves; the matrices inverted here have a
+// positive-definite anti-Hermitian part, so A11 and S stay invertible.
+static int run_inverse(skkr_ctx* c, const InvArgs& a, int nb) {
+  if (nb <= 0) return SKKR_OK;
+  if (a.n <= 128) return launch_gjinv(c, a, nb);
+  if (a.n > 256 || a.mode != 0 || !c->Asc || nb > 2 * c->B || a.n != c->n) return SKKR_ERR_UNSUPPORTED;
+  const int n = a.n, h1 = 128, h2 = n - 128;
+  const z one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  {
+    dim3 grid((unsigned)(((long long)n * n + 255) / 256), nb);
+    assemble_kernel<<<grid, 256, 0, c->stream>>>(a, c->p_Asc.d);
+    HIPCHK(hipGetLastError());
+    const long long off[4] = {0, h1, (long long)h1 * n, h1 + (long long)h1 * n};
+    for (int v = 0; v < 4; ++v)
+      ptr_offset_kernel<<<(nb + 255) / 256, 256, 0, c->stream>>>(nb, a.out, off[v], c->dv_out[v]);
+    HIPCHK(hipGetLastError());
+    c->launches += 5;
+  }
+  z** B11 = c->dv_out[0];
+  z** B21 = c->dv_out[1];
+  z** B12 = c->dv_out[2];
+  z** B22 = c->dv_out[3];
+  InvArgs i1;
+  memset(&i1, 0, sizeof(i1));
+  i1.n = h1;
+  i1.lk = h1;
+  i1.ld = n;
+  i1.ldo = n;
+  i1.base = c->p_Asc.d;
+  i1.out = B11;
+  i1.active = a.active;
+  i1.status = a.status;
+  CHK(launch_gjinv(c, i1, nb));
+  CHK(launch_zgemm(c, h2, h1, h1, c->p_A21.d, n, B11, n, c->p_Csc.d, h2, one, zero, nb, a.active, nullptr));
+  CHK(launch_zgemm(c, h2, h2, h1, c->p_Csc.d, h2, c->p_A12.d, n, c->p_A22.d, n, mone, one, nb, a.active, nullptr));
+  InvArgs i2 = i1;
+  i2.n = h2;
+  i2.lk = h2;
+  i2.base = c->p_A22.d;
+  i2.out = B22;
+  CHK(launch_gjinv(c, i2, nb));
+  CHK(launch_zgemm(c, h1, h2, h1, B11, n, c->p_A12.d, n, c->p_Dsc.d, h1, one, zero, nb, a.active, nullptr));
+  CHK(launch_zgemm(c, h1, h2, h2, c->p_Dsc.d, h1, B22, n, B12, n, mone, zero, nb, a.active, nullptr));
+  CHK(launch_zgemm(c, h2, h1, h2, B22, n, c->p_Csc.d, h2, B21, n, mone, zero, nb, a.active, nullptr));
+  CHK(launch_zgemm(c, h1, h1, h2, B12, n, c->p_Csc.d, h2, B11, n, mone, one, nb, a.active, nullptr));
+  return SKKR_OK;
+}
+
 static InvArgs inv_args(skkr_ctx* c) {
   InvArgs a;
   memset(&a, 0, sizeof(a));
@@ -354,7 +427,7 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
     ia.out = c->p_Yinv.d;
     ia.active = c->active;
     ia.status = c->status;
-    CHK(launch_gjinv(c, ia, B2));
+    CHK(run_inverse(c, ia, B2));
     CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
                      one, zero, B2, c->active));
     EpiArgs e;
@@ -390,7 +463,7 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
   ia.out = c->p_Yinv.d;
   ia.active = c->active;
   ia.status = c->status;
-  CHK(launch_gjinv(c, ia, B2));
+  CHK(run_inverse(c, ia, B2));
   if (want_sigma) {
     // T = ml g_s (into AYBY as n x n), sigma = T mr
     CHK(launch_zgemm(c, n, n, n, c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero, B2,
@@ -427,7 +500,7 @@ static int run_sweeps(skkr_ctx* c, int nb) {
     ia.sub1 = c->p_Y.d;
     ia.out = c->p_G[p].d;
     ia.status = c->status_sw;
-    CHK(launch_gjinv(c, ia, nb));
+    CHK(run_inverse(c, ia, nb));
     CHK(launch_zgemm(c, n, n, n, c->p_G[p].d, n, c->p_M01.d, n, c->p_Zs[p].d, n, one, zero, nb,
                      nullptr));
     CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_Zs[p].d, n, c->p_Y.d, n, one, zero, nb,
@@ -441,7 +514,7 @@ static int run_sweeps(skkr_ctx* c, int nb) {
     ia.sub2 = c->p_Y.d;
     ia.out = c->p_G[P - 1].d;
     ia.status = c->status_sw;
-    CHK(launch_gjinv(c, ia, nb));
+    CHK(run_inverse(c, ia, nb));
     CHK(launch_extract(c, P - 1, nb));
   }
   for (int p = P - 2; p >= 0; --p) {
@@ -503,7 +576,7 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   c->n = cfg->lk * cfg->sites_per_pl;
   c->nn = c->n * c->n;
   c->ns = cfg->sites_per_pl;
-  if (c->n > 128) {
+  if (c->n > 256) {
     delete c;
     return SKKR_ERR_UNSUPPORTED;
   }
@@ -580,8 +653,9 @@ int skkr_set_tinv(skkr_ctx* c, const double* ti, const double* tl, const double*
 static int auto_batch(skkr_ctx* c, int nk) {
   int B = c->cfg.batch;
   if (B <= 0) {
-    // ~ 28 n^2 + 2P n^2 complex per k; keep a batch under ~24 GB
-    double per_k = (28.0 + 2.0 * c->cfg.n_pl) * c->nn * 16.0;
+    // ~ 28 n^2 + 2P n^2 complex per k (+ 3 n^2 block-inverse scratch for
+    // n > 128); keep a batch under ~24 GB
+    double per_k = (31.0 + 2.0 * c->cfg.n_pl) * c->nn * 16.0;
     B = (int)std::min(1024.0, std::max(1.0, 24e9 / per_k));
   }
   return std::max(1, std::min(B, nk));
@@ -732,7 +806,8 @@ struct TmpMats {
 };
 
 int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
-  if (!c || n <= 0 || n > 128 || nb <= 0 || !a) return SKKR_ERR_ARG;
+  if (!c || n <= 0 || n > 256 || nb <= 0 || !a) return SKKR_ERR_ARG;
+  if (n > 128 && n != c->n) return SKKR_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->cfg.device));
   TmpMats m, st;
   CHK(m.init(nb, (size_t)n * n, a, c->stream));
@@ -746,7 +821,8 @@ int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
   ia.base = m.p;
   ia.out = m.p;
   ia.status = dst;
-  int r = launch_gjinv(c, ia, nb);
+  if (n > 128) CHK(ensure_batch(c, std::max(c->B, (nb + 1) / 2)));
+  int r = run_inverse(c, ia, nb);
   std::vector<int> hs(nb);
   if (r == SKKR_OK) {
     HIPCHK(hipMemcpyAsync(a, m.d, sizeof(z) * n * n * nb, hipMemcpyDeviceToHost, c->stream));

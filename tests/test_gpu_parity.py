@@ -219,3 +219,36 @@ def test_itermax_exceeded_is_reported(gpu):
     ref = oracle.tau_bz(sy, E, k, w, itermax=1)
     assert ref["rc"] == 3
     np.testing.assert_array_equal(r["iters"], ref["iters"])
+
+
+@pytest.mark.parametrize("n", [144, 200, 256])
+def test_block_inverse_beyond_128(gpu, n):
+    """n > 128: 2x2 block elimination around the leading 128 x 128 block, for
+    matrices with a positive-definite anti-Hermitian part (E - H, Im E > 0)."""
+    rng = np.random.default_rng(n)
+    h = rng.normal(size=(4, n, n)) + 1j * rng.normal(size=(4, n, n))
+    h = (h + np.conj(np.swapaxes(h, 1, 2))) / 2
+    a = (0.3 + 0.05j) * np.eye(n) - h
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        inv, st = ctx.gjinv(a)
+    assert (st == 0).all()
+    ref = np.linalg.inv(a)
+    assert rel(inv, ref) < 1e-11
+
+
+def test_config3_sample_matches_oracle(gpu):
+    """Config 3 (BdG, lk=64, n=256, 12 PLs) on 4 k-points at one contour energy."""
+    from skkr_budapest_amd import contour
+    cfg = S.CONFIGS[3]
+    sy = S.make_system(cfg)
+    z, _ = contour.config_contour()
+    E = z[13]
+    k, w = S.uniform_kmesh(16)
+    idx = np.array([0, 77, 130, 255])
+    ks, ws = k[idx], w[idx]
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=4)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, ks, ws)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL_CFG1
