@@ -24,6 +24,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("BZ-integrated tau k-points/s per energy: mixed+refine vs FP64, "
           "at 1e-8 rel err")
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes moved per algorithmic byte by zgemm_f64_kernel<128,128,16>: rocprofv3
+# FETCH_SIZE + WRITE_SIZE of 1024 x complex 128^3 products, calibrated with a
+# plain 256 MiB copy (profiles/hbm_traffic_pmc.md): 805.8 MB / 805.3 MB
+GEMM_TRAFFIC_PER_ALGO_BYTE = 805.8 / 805.3
 
 
 def algorithmic_flops_per_k(n, P, it_l, it_r):
@@ -88,6 +92,7 @@ def main():
     dom_launch_ms = sum(s["dom_ms"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
     dom_flops = sum(s["dom_flops"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
     achieved = dom_flops / (dom_launch_ms * 1e-3) / 1e12 if dom_launch_ms > 0 else 0.0
+    dom_bytes = sum(s["dom_bytes"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
     value = nk / (ms * 1e-3)
     out = {
         "metric": METRIC,
@@ -112,7 +117,10 @@ def main():
                                 ("fourier", "decimation", "sweeps", "accumulate")}},
         "roofline": {"bound": "mfma", "kernel": st["dom_name"], "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": dom_bytes * GEMM_TRAFFIC_PER_ALGO_BYTE,
+                     "traffic_unit": "bytes per launch",
+                     "algorithmic_flops_per_launch": dom_flops,
                      "avg_launch_us": dom_launch_ms * 1e3},
         "cpu_baseline": None,
     }

@@ -80,6 +80,7 @@ typedef struct {
   double dom_ms;
   double dom_flops;       /* algorithmic flops of those launches */
   char dom_name[32];
+  double dom_bytes;       /* algorithmic HBM bytes of those launches (A, B read, C written) */
 } skkr_stats;
 
 /* Context: one per host thread / rank; calls on a context are serialised on

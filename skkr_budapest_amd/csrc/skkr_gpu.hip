@@ -95,9 +95,9 @@ struct skkr_ctx {
   long long launches = 0;
   int timing_detail = 0;
   std::vector<hipEvent_t> evpool;  // pairs bracketing each dominant-kernel launch
-  std::vector<double> evflops;
+  std::vector<double> evflops, evbytes;
   size_t evused = 0;
-  double dom_ms = 0.0, dom_flops = 0.0;
+  double dom_ms = 0.0, dom_flops = 0.0, dom_bytes = 0.0;
   long long dom_launches = 0;
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 128
@@ -236,10 +236,11 @@ static void dom_begin(skkr_ctx* c) {
   }
   hipEventRecord(c->evpool[c->evused], c->stream);
 }
-static void dom_end(skkr_ctx* c, double flops) {
+static void dom_end(skkr_ctx* c, double flops, double bytes) {
   if (!c->timing_detail) return;
   hipEventRecord(c->evpool[c->evused + 1], c->stream);
   c->evflops.push_back(flops);
+  c->evbytes.push_back(bytes);
   c->evused += 2;
 }
 // call after the stream was synchronised
@@ -249,10 +250,12 @@ static void dom_collect(skkr_ctx* c) {
     hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]);
     c->dom_ms += ms;
     c->dom_flops += c->evflops[i / 2];
+    c->dom_bytes += c->evbytes[i / 2];
     c->dom_launches++;
   }
   c->evused = 0;
   c->evflops.clear();
+  c->evbytes.clear();
 }
 
 // Work-group tile of a GEMM shape: the context's tuned choice (skkr_autotune
@@ -285,7 +288,8 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
     zgemm_f64_kernel<32, 32, 16><<<grid, 64, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   HIPCHK(hipGetLastError());
-  dom_end(c, 8.0 * m * n * (double)k * nb);
+  dom_end(c, 8.0 * m * n * (double)k * nb,
+          16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
   c->launches++;
   return SKKR_OK;
 }
@@ -678,8 +682,10 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
   c->dom_ms = 0;
   c->dom_flops = 0;
   c->dom_launches = 0;
+  c->dom_bytes = 0;
   c->evused = 0;
   c->evflops.clear();
+  c->evbytes.clear();
   c->launches = 0;
   HIPCHK(hipMemsetAsync(c->taubz, 0, sizeof(z) * tsz, c->stream));
   hipEvent_t ev[5];
@@ -761,6 +767,7 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
     st->dom_launches = c->dom_launches;
     st->dom_ms = c->dom_ms;
     st->dom_flops = c->dom_flops;
+    st->dom_bytes = c->dom_bytes;
     snprintf(st->dom_name, sizeof(st->dom_name), "zgemm_f64_kernel");
   }
   c->timing_detail = 0;

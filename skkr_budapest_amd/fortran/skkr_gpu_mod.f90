@@ -27,6 +27,7 @@ module skkr_gpu
      integer(c_long_long) :: dom_launches
      real(c_double) :: dom_ms, dom_flops
      character(kind=c_char) :: dom_name(32)
+     real(c_double) :: dom_bytes
   end type skkr_stats
 
   interface

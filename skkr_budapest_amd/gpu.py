@@ -57,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("max_iter", ctypes.c_int), ("n_batches", ctypes.c_int),
                 ("n_launches", ctypes.c_longlong), ("timing_detail", ctypes.c_int),
                 ("dom_launches", ctypes.c_longlong), ("dom_ms", ctypes.c_double),
-                ("dom_flops", ctypes.c_double), ("dom_name", ctypes.c_char * 32)]
+                ("dom_flops", ctypes.c_double), ("dom_name", ctypes.c_char * 32),
+                ("dom_bytes", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -1,0 +1,58 @@
+"""Small workloads for rocprofv3 runs of single kernels.
+
+  python -m skkr_budapest_amd.profiling copy     # plain device copy, 256 MiB
+  python -m skkr_budapest_amd.profiling gemm     # 1024 complex GEMMs 128^3
+  python -m skkr_budapest_amd.profiling inverse  # 1024 inversions 128 x 128
+
+The copy moves a known number of bytes, so its FETCH_SIZE / WRITE_SIZE
+counters calibrate those of the other kernels.  Each workload runs 3 times.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+COPY_BYTES = 256 << 20
+
+
+def run_copy():
+    import torch
+    src = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy: {COPY_BYTES} bytes read and written per dispatch", flush=True)
+
+
+def _mats(n, nb, seed):
+    rng = np.random.default_rng(seed)
+    h = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    return h
+
+
+def run_gemm(n=128, nb=1024):
+    from . import gpu
+    a, b = _mats(n, nb, 1), _mats(n, nb, 2)
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        for _ in range(3):
+            ctx.zgemm(a, b)
+    print(f"gemm: {nb} x complex {n}^3, algorithmic bytes per dispatch "
+          f"{3 * nb * n * n * 16} (A, B read, C written)", flush=True)
+
+
+def run_inverse(n=128, nb=1024):
+    from . import gpu
+    h = _mats(n, nb, 3)
+    a = (0.3 + 0.05j) * np.eye(n) - (h + np.conj(np.swapaxes(h, 1, 2))) / 2
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        for _ in range(3):
+            ctx.gjinv(a)
+    print(f"inverse: {nb} x {n} x {n}, algorithmic flops per dispatch {8 * n ** 3 * nb}",
+          flush=True)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "gemm"
+    {"copy": run_copy, "gemm": run_gemm, "inverse": run_inverse}[what]()
