@@ -123,7 +123,8 @@ int skkr_pl_blocks(skkr_ctx *ctx, int nk, const double *k, double *m00,
                    double *m01, double *m10);
 
 /* In-place inverse of nb matrices n x n (gjinv, skkr_kpar_v6/main/gjinv.f:2);
- * n <= 128.  status (nb, optional) gets SKKR_ERR_SINGULAR per matrix. */
+ * n <= 128, or 128 < n <= 256 when n equals the context's lk*sites_per_pl.
+ * status (nb, optional) gets SKKR_ERR_SINGULAR per matrix. */
 int skkr_gjinv_batch(skkr_ctx *ctx, int n, int nb, double *a, int *status);
 
 /* c = a*b for nb pairs of n x n matrices (doubmt2, matops.f:204-228). */
@@ -144,9 +145,11 @@ int skkr_dugo_batch(skkr_ctx *ctx, char side, int nb, const double *m00,
                     const double *m01, const double *m10, double *xy,
                     int *iters, int *status);
 
-/* Godfrin sweeps for nb k-points (tau2d, skkr_kpar_v6/main/tau2d.f:2) with
- * the interface t^-1 set by skkr_set_tinv: tau (nb x sites x lk x lk) =
- * w * site-diagonal blocks of the PL-diagonal tau. */
+/* PL-diagonal tau for nb k-points (replaces tau2d, skkr_kpar_v6/main/tau2d.f:2)
+ * with the interface t^-1 set by skkr_set_tinv and the lead self-energies
+ * x = X_P, y = Y_1: tau (nb x sites x lk x lk) = w * site-diagonal blocks of
+ * (t^-1 - G)^-1's PL-diagonal blocks (the same blocks as the Godfrin sweeps,
+ * computed in recursive Green's-function form). */
 int skkr_tau2d_batch(skkr_ctx *ctx, int nb, const double *m00,
                      const double *m01, const double *m10, const double *x,
                      const double *y, double w, double *tau, int *status);
