@@ -114,6 +114,25 @@ int skkr_tau_bz(skkr_ctx *ctx, double e_re, double e_im, int nk,
                 double *tr_tau_k, int *iters_k, int *status_k,
                 skkr_stats *stats);
 
+/* Options of a context (no reference counterpart; execution choices that
+ * leave FP64 results bitwise unchanged):
+ *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it
+ *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 128
+ *   "inv_kernel" 0 blocked MFMA Gauss-Jordan | 1 row-wise Gauss-Jordan
+ *                (different rounding; not bitwise equal to 0) */
+int skkr_set_option(skkr_ctx *ctx, const char *key, int value);
+
+typedef struct {
+  int tile_nn;          /* chosen work-group tile for n x n x n products */
+  int tile_2nn;         /* chosen tile for the 2n x n x n decimation products */
+  double us_nn[3];      /* mean launch time, tiles 32 / 64 / 128 (0 = not run) */
+  double us_2nn[3];
+} skkr_tuning;
+
+/* Time the GEMM tile variants on this context's batch shapes (device buffers
+ * of a full batch, contents irrelevant) and keep the fastest per shape. */
+int skkr_autotune(skkr_ctx *ctx, skkr_tuning *out);
+
 /* ---- per-routine batched entry points (host arrays in and out) ---------- */
 
 /* G(k) -> PL blocks for nk k-points (the gstore+packer step,

@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/skkr_gpu.h"
@@ -101,6 +102,10 @@ struct skkr_ctx {
   long long dom_launches = 0;
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 128
+  int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
+  int graphs = 0;
+  bool capturing = false;
+  std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by batch size nb
 };
 
 // ------------------------------------------------------------------------
@@ -129,8 +134,14 @@ static int free_batch(skkr_ctx* c) {
   return SKKR_OK;
 }
 
+static void drop_graphs(skkr_ctx* c) {
+  for (auto& kv : c->sweep_graphs) hipGraphExecDestroy(kv.second);
+  c->sweep_graphs.clear();
+}
+
 static int ensure_batch(skkr_ctx* c, int B) {
   if (c->B >= B) return SKKR_OK;
+  drop_graphs(c);
   free_batch(c);
   const int n = c->n;
   const size_t nn = c->nn, P = c->cfg.n_pl, lk = c->cfg.lk, ns = c->ns;
@@ -228,7 +239,7 @@ static int ensure_batch(skkr_ctx* c, int B) {
 // Dominant-kernel timing: HIP events recorded on the context stream around
 // each launch (no synchronisation); read back when the stream has drained.
 static void dom_begin(skkr_ctx* c) {
-  if (!c->timing_detail) return;
+  if (!c->timing_detail || c->capturing) return;
   while (c->evpool.size() < c->evused + 2) {
     hipEvent_t e;
     hipEventCreate(&e);
@@ -237,7 +248,7 @@ static void dom_begin(skkr_ctx* c) {
   hipEventRecord(c->evpool[c->evused], c->stream);
 }
 static void dom_end(skkr_ctx* c, double flops, double bytes) {
-  if (!c->timing_detail) return;
+  if (!c->timing_detail || c->capturing) return;
   hipEventRecord(c->evpool[c->evused + 1], c->stream);
   c->evflops.push_back(flops);
   c->evbytes.push_back(bytes);
@@ -262,6 +273,8 @@ static void dom_collect(skkr_ctx* c) {
 // or SKKR_GEMM_TILE), else the largest tile that the shape fills.
 static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   int t = (m >= 128 && n >= 128) ? 128 : ((m > 32 || n > 32) ? 64 : 32);
+  if (c && c->tile_nn && m == c->n && n == c->n) t = c->tile_nn;
+  if (c && c->tile_2nn && m == 2 * c->n && n == c->n) t = c->tile_2nn;
   if (c && c->gemm_tile_force > 0) t = c->gemm_tile_force;
   *tm = t;
   *tiles = ((m + t - 1) / t) * ((n + t - 1) / t);
@@ -493,7 +506,7 @@ static int launch_extract(skkr_ctx* c, int p, int nb) {
   return SKKR_OK;
 }
 
-static int run_sweeps(skkr_ctx* c, int nb) {
+static int sweep_launches(skkr_ctx* c, int nb) {
   const int n = c->n, P = c->cfg.n_pl, s = c->cfg.sites_per_pl, lk = c->cfg.lk;
   const size_t tblk = (size_t)s * lk * lk;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
@@ -530,6 +543,33 @@ static int run_sweeps(skkr_ctx* c, int nb) {
                      nullptr));
     CHK(launch_extract(c, p, nb));
   }
+  return SKKR_OK;
+}
+
+// The sweeps of a batch, optionally as a captured HIP graph (one per nb; all
+// buffers and pointer arrays are fixed for the context's batch size).
+static int run_sweeps(skkr_ctx* c, int nb) {
+  if (!c->graphs) return sweep_launches(c, nb);
+  auto it = c->sweep_graphs.find(nb);
+  if (it == c->sweep_graphs.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    const int r = sweep_launches(c, nb);
+    c->capturing = false;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SKKR_OK) {
+      if (g) hipGraphDestroy(g);
+      return r;
+    }
+    HIPCHK(e);
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(ei);
+    it = c->sweep_graphs.emplace(nb, ge).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
   return SKKR_OK;
 }
 
@@ -605,6 +645,7 @@ void skkr_destroy(skkr_ctx* c) {
   if (!c) return;
   hipSetDevice(c->cfg.device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (void* p : c->allocs) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
@@ -772,6 +813,81 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
   }
   c->timing_detail = 0;
   return rc;
+}
+
+int skkr_set_option(skkr_ctx* c, const char* key, int value) {
+  if (!c || !key) return SKKR_ERR_ARG;
+  if (strcmp(key, "graphs") == 0) {
+    if (value != 0 && value != 1) return SKKR_ERR_ARG;
+    c->graphs = value;
+    return SKKR_OK;
+  }
+  if (strcmp(key, "gemm_tile") == 0) {
+    if (value != 0 && value != 32 && value != 64 && value != 128) return SKKR_ERR_ARG;
+    c->gemm_tile_force = value;
+    drop_graphs(c);
+    return SKKR_OK;
+  }
+  if (strcmp(key, "inv_kernel") == 0) {
+    if (value != 0 && value != 1) return SKKR_ERR_ARG;
+    c->inv_kernel = value;
+    drop_graphs(c);
+    return SKKR_OK;
+  }
+  return SKKR_ERR_ARG;
+}
+
+// mean launch time (us) of the plain GEMM m x n x k over nb batch entries
+static int time_gemm(skkr_ctx* c, int tile, int m, int n, int k, z** A, int lda, z** B, int ldb,
+                     z** C, int ldc, int nb, double* us) {
+  const int saved = c->gemm_tile_force;
+  c->gemm_tile_force = tile;
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int r = launch_zgemm(c, m, n, k, A, lda, B, ldb, C, ldc, one, zero, nb, nullptr);
+  const int reps = 3;
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps && r == SKKR_OK; ++i)
+    r = launch_zgemm(c, m, n, k, A, lda, B, ldb, C, ldc, one, zero, nb, nullptr);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->gemm_tile_force = saved;
+  *us = 1e3 * ms / reps;
+  return r;
+}
+
+int skkr_autotune(skkr_ctx* c, skkr_tuning* out) {
+  if (!c) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int B = c->cfg.batch > 0 ? c->cfg.batch : 1024;
+  CHK(ensure_batch(c, std::max(c->B, std::min(B, 1024))));
+  const int n = c->n, nb = c->B;
+  const int tiles[3] = {32, 64, 128};
+  skkr_tuning t;
+  memset(&t, 0, sizeof(t));
+  double best_nn = 1e30, best_2nn = 1e30;
+  for (int v = 0; v < 3; ++v) {
+    if (tiles[v] == 32 && n > 64) continue;  // far too many work-groups to matter
+    double us = 0.0;
+    CHK(time_gemm(c, tiles[v], n, n, n, c->p_M10.d, n, c->p_M01.d, n, c->p_W.d, n, nb, &us));
+    t.us_nn[v] = us;
+    if (us < best_nn) { best_nn = us; t.tile_nn = tiles[v]; }
+    CHK(time_gemm(c, tiles[v], 2 * n, n, n, c->p_AB[0].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d,
+                  2 * n, 2 * nb, &us));
+    t.us_2nn[v] = us;
+    if (us < best_2nn) { best_2nn = us; t.tile_2nn = tiles[v]; }
+  }
+  c->tile_nn = t.tile_nn;
+  c->tile_2nn = t.tile_2nn;
+  drop_graphs(c);
+  if (out) *out = t;
+  return SKKR_OK;
 }
 
 // ---------------- per-routine batched entry points ----------------------

@@ -32,7 +32,7 @@ FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
 SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structure",
            "skkr_set_tinv", "skkr_tau_bz", "skkr_pl_blocks", "skkr_gjinv_batch",
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
-           "skkr_tau2d_batch")
+           "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune")
 
 
 class SkkrError(RuntimeError):
@@ -66,6 +66,15 @@ class Stats(ctypes.Structure):
         return d
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_nn", ctypes.c_int), ("tile_2nn", ctypes.c_int),
+                ("us_nn", ctypes.c_double * 3), ("us_2nn", ctypes.c_double * 3)]
+
+    def as_dict(self):
+        return {"tile_nn": self.tile_nn, "tile_2nn": self.tile_2nn,
+                "us_nn": list(self.us_nn), "us_2nn": list(self.us_2nn)}
+
+
 _LIB = None
 
 
@@ -90,6 +99,8 @@ def lib():
         L.skkr_surfgfd_batch.argtypes = [V, I, V, V, V, V, V, V]
         L.skkr_dugo_batch.argtypes = [V, ctypes.c_char, I, V, V, V, V, V, V]
         L.skkr_tau2d_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
+        L.skkr_set_option.argtypes = [V, ctypes.c_char_p, I]
+        L.skkr_autotune.argtypes = [V, ctypes.POINTER(Tuning)]
         _LIB = L
     return _LIB
 
@@ -145,6 +156,15 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_option(self, key, value):
+        self._check(lib().skkr_set_option(self._h, key.encode(), int(value)),
+                    f"skkr_set_option({key})")
+
+    def autotune(self):
+        t = Tuning()
+        self._check(lib().skkr_autotune(self._h, ctypes.byref(t)), "skkr_autotune")
+        return t.as_dict()
 
     # ---- k-loop -----------------------------------------------------------
     def set_structure(self, R, G_intra, G_inter):

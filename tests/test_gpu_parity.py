@@ -252,3 +252,41 @@ def test_config3_sample_matches_oracle(gpu):
         r = ctx.tau_bz(E, ks, ws)
     np.testing.assert_array_equal(r["iters"], ref["iters"])
     assert rel(r["tau_bz"], ref["tau_bz"]) < TOL_CFG1
+
+
+def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
+    """HIP-graph replays of the sweeps, forced GEMM tiles and the autotuned
+    choice all give the same FP64 bits as the eager default."""
+    sy = small_sys(lk=16, s=4, P=6, r2max=4, E=0.4 + 0.05j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(6)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        base = ctx.tau_bz(E, k, w)
+        ctx.set_option("graphs", 1)
+        for rep in range(3):
+            r = ctx.tau_bz(E, k, w)
+            np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+            np.testing.assert_array_equal(r["trk"], base["trk"])
+            # eager work on another batch size between replays
+            ctx.set_option("graphs", 0)
+            ctx.tau_bz(E, k[:7], w[:7])
+            ctx.set_option("graphs", 1)
+        ctx.set_option("graphs", 0)
+        for tile in (32, 64, 128):
+            ctx.set_option("gemm_tile", tile)
+            r = ctx.tau_bz(E, k, w)
+            np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+        ctx.set_option("gemm_tile", 0)
+        t = ctx.autotune()
+        assert t["tile_nn"] in (32, 64, 128) and t["tile_2nn"] in (32, 64, 128)
+        r = ctx.tau_bz(E, k, w)
+        np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+
+
+def test_set_option_rejects_unknown(gpu):
+    with gpu.Context(lk=4, sites_per_pl=1, n_pl=1) as ctx:
+        with pytest.raises(gpu.SkkrError):
+            ctx.set_option("no_such_option", 1)
+        with pytest.raises(gpu.SkkrError):
+            ctx.set_option("gemm_tile", 48)
