@@ -8,6 +8,11 @@ Default workload: the relativistic layer system with lk=32, 4 sites per
 principal layer (n=128), 20 interface PLs, 25 in-plane vectors, one energy
 E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points).
 
+--config 2 / 3 run a 16-point contour instead (a step is the whole contour):
+config 2 the same system with a 64x64 mesh per energy, config 3 the BdG
+system (lk=64, n=256, 12 PLs) with 256 ... 16384 k per energy, geometric in
+Im E.  --config 0 is the small nonrelativistic system (16x16 mesh).
+
 Prints one JSON line (rank 0).
 """
 import argparse
@@ -48,6 +53,47 @@ def cpu_baseline(sy, E, k, w):
                       f"{nworkers} threads, round-robin k split, {dt:.1f} s"}
 
 
+def contour_bench(args, cfg, sy):
+    from skkr_budapest_amd import contour, energy_loop, gpu
+    z, wz = contour.config_contour()
+    if args.config == 2:
+        sides = [cfg.mesh] * len(z)
+    else:
+        sides = energy_loop.geometric_mesh_sides(z.imag, 256, 16384)
+    nk_total = sum(s * s for s in sides)
+    print(f"bench: config {args.config} contour {len(z)} energies, {nk_total} k in total",
+          file=sys.stderr, flush=True)
+    ctx = gpu.Context.for_system(sy)
+    for i in range(args.warmup):
+        energy_loop.run_contour(ctx, sy, z[-1:], wz[-1:], sides[-1:])
+    times = []
+    recs = None
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        recs, total = energy_loop.run_contour(ctx, sy, z, wz, sides)
+        times.append(time.perf_counter() - t0)
+        print(f"bench: contour step {i} {times[-1]:.2f} s, integral of tr G {total}",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    sec = float(np.mean(times))
+    out = {
+        "metric": METRIC, "value": nk_total / sec, "unit": "k-points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded G_r(R), t^-1)",
+        "config": {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} n={cfg.n} "
+                               f"N_R={cfg.n_R}, 16-point contour E_b=-0.8 E_F=0.6, "
+                               f"{nk_total} k in total",
+                   "precision": "fp64",
+                   "integrated_trG": [total.real, total.imag],
+                   "energies": [{"E": [r["E"].real, r["E"].imag], "nk": r["nk"],
+                                 "seconds": r["seconds"], "dos": r["dos"],
+                                 "mean_iter": float(r["iters"].mean())} for r in recs]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +113,8 @@ def main():
 
     cfg = S.CONFIGS[args.config]
     sy = S.make_system(cfg)
+    if args.config in (2, 3):
+        return contour_bench(args, cfg, sy)
     E = cfg.energies[0]
     k, w = S.uniform_kmesh(cfg.mesh)
     nk = len(w)
