@@ -34,6 +34,19 @@ def build_hip(verbose=False):
     return out
 
 
+def build_mlp(verbose=False):
+    out = os.path.join(HERE, "libskkr_mlp.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "mlp.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_fortran(verbose=False):
     """Fortran host: ISO_C_BINDING module + cpacoord-style k-loop driver."""
     fc = shutil.which("flang") or shutil.which("amdflang")
@@ -58,4 +71,5 @@ def build_fortran(verbose=False):
 
 def build_all(verbose=False):
     build_hip(verbose)
+    build_mlp(verbose)
     build_fortran(verbose)

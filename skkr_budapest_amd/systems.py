@@ -83,7 +83,7 @@ def from_colmajor(buf: np.ndarray) -> np.ndarray:
 @dataclasses.dataclass
 class Config:
     name: str
-    cfg_id: int
+    cfg_id: int             # seed id of the random system
     lk: int                 # site block width
     s: int                  # sites per principal layer
     P: int                  # interface principal layers
@@ -114,10 +114,11 @@ CONFIGS = {
               spin_diag=True, energies=(0.5 + 0.05j,), mesh=16),
     1: Config("cfg1-rel-lmax3", 1, lk=32, s=4, P=20, r2max=8, tmodel="herm",
               energies=(0.6 + 0.02j,), mesh=64),
-    2: Config("cfg2-rel-contour", 2, lk=32, s=4, P=20, r2max=8, tmodel="herm",
+    # configs 2 and 4 run the config-1 system (same seed id)
+    2: Config("cfg2-rel-contour", 1, lk=32, s=4, P=20, r2max=8, tmodel="herm",
               mesh=64),
     3: Config("cfg3-bdg", 3, lk=64, s=4, P=12, r2max=8, tmodel="bdg"),
-    4: Config("cfg4-rel-adaptive", 4, lk=32, s=4, P=20, r2max=8, tmodel="herm",
+    4: Config("cfg4-rel-adaptive", 1, lk=32, s=4, P=20, r2max=8, tmodel="herm",
               energies=(0.6 + 0.005j,), mesh=32),
 }
 

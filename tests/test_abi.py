@@ -45,3 +45,32 @@ def test_argument_validation_without_device():
     assert lib.skkr_create(None, None) == 1          # SKKR_ERR_ARG
     lib.skkr_error_string.restype = ctypes.c_char_p
     assert lib.skkr_error_string(6) == b"decimation exceeded itermax"
+
+
+MLP_HEADER = os.path.join(ROOT, "include", "skkr_mlp.h")
+MLP_LIB = os.path.join(ROOT, "skkr_budapest_amd", "libskkr_mlp.so")
+
+
+def _declared_in(path):
+    src = re.sub(r"/\*.*?\*/", "", open(path).read(), flags=re.S)
+    return sorted(set(re.findall(r"\b(skkr_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.mark.skipif(not os.path.exists(MLP_LIB), reason="library not built")
+def test_mlp_library_exports_every_declared_symbol():
+    d = _declared_in(MLP_HEADER)
+    assert "skkr_mlp_train" in d and "skkr_mlp_loss_grad" in d
+    lib = ctypes.CDLL(MLP_LIB)
+    missing = [s for s in d if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(MLP_LIB), reason="library not built")
+def test_mlp_argument_validation_without_device():
+    lib = ctypes.CDLL(MLP_LIB)
+    lib.skkr_mlp_create.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_ulonglong]
+    h = ctypes.c_void_p()
+    assert lib.skkr_mlp_create(ctypes.byref(h), 0, 64, 0, 1) == 1     # nin < 1
+    assert lib.skkr_mlp_create(ctypes.byref(h), 8, 32, 0, 1) == 1     # nhid != 64
+    assert lib.skkr_mlp_create(None, 8, 64, 0, 1) == 1

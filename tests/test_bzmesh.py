@@ -1,0 +1,110 @@
+"""Host logic of the adaptive BZ mesh (skkr_budapest_amd/bzmesh.py), pinned by
+the closed form of the square-lattice Green's function
+    (1/4pi^2) int int dk / (z - cos kx - cos ky) = (2 / (pi z)) K(4 / z^2)
+(K the complete elliptic integral of the first kind, parameter m = k^2):
+smooth for z = 3, sharply peaked at k = 0 for z -> 2+."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import ellipk
+
+from skkr_budapest_amd import bzmesh as BM
+
+
+def lgf(z):
+    return 2.0 / (math.pi * z) * ellipk(4.0 / z ** 2)
+
+
+def lgf_integrand(z):
+    return lambda k: 1.0 / (z - np.cos(k[:, 0]) - np.cos(k[:, 1])) + 0j
+
+
+@pytest.mark.parametrize("npts,ncell", [(4, 4), (8, 4), (64, 32), (256, 256), (512, 256)])
+def test_bumps_are_a_partition_of_unity(npts, ncell):
+    b = BM._bump_matrix(npts, ncell, 1.0, 6.0)
+    assert np.allclose(b.sum(1), 1.0, atol=1e-14)
+    assert b.min() >= -1e-15
+
+
+def test_closed_form_against_a_fine_uniform_mesh():
+    n = 256
+    g = -math.pi + 2 * math.pi * np.arange(n) / n
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    k = np.stack([X.ravel(), Y.ravel()], 1)
+    assert abs(lgf_integrand(2.2)(k).mean() - lgf(2.2)) < 1e-14 * lgf(2.2)
+
+
+def test_uniform_levels_match_the_trapezoid_rule_and_the_convergence_test():
+    ev = lgf_integrand(2.05)
+    hist = BM.uniform_levels(ev, 8, 5)
+    assert [n for _, n in hist] == [(8 << l) ** 2 for l in range(6)]   # each point once
+    errs = [abs(I - lgf(2.05)) / lgf(2.05) for I, _ in hist]
+    assert errs[-1] < 1e-14 and errs[2] > 1e-6
+    # the convergence test stops at the first level agreeing with the previous to tol
+    conv = BM.uniform_levels(ev, 8, 5, tol=1e-6)
+    d = [abs(hist[l][0] - hist[l - 1][0]) / abs(hist[l][0]) for l in range(1, 6)]
+    first = 1 + next(i for i, x in enumerate(d) if x <= 1e-6)
+    assert len(conv) == first + 1
+
+
+def test_windowed_rule_is_exact_for_any_proposal():
+    """The window identity holds for any set of refined cells: with a smooth
+    integrand every level resolves its piece, so random proposals all give
+    the integral to the aliasing of the windows themselves (a Gaussian step of
+    width sigma = 1 cell aliases at exp(-2 pi^2 sigma^2) ~ 3e-9 of its height)."""
+    ev = lgf_integrand(3.0)
+    rng = np.random.default_rng(4)
+    for trial in range(3):
+        m = BM.WindowedMesh(ev, n0=16, lmax=3, im_e=0.01)
+        r = m.integrate("learned", theta=1e-8,
+                        predict=lambda x: rng.uniform(-12, -4, size=len(x)), margin=0.0)
+        assert r["levels"] >= 1 and sum(r["flagged"]) > 0
+        assert abs(r["integral"] - lgf(3.0)) < 1e-8 * lgf(3.0)
+
+
+def test_uniform_proposal_reproduces_uniform_refinement():
+    ev = lgf_integrand(2.05)
+    m = BM.WindowedMesh(ev, n0=8, lmax=4, im_e=0.01)
+    r = m.integrate("uniform")
+    I4, n4 = BM.uniform_levels(ev, 8, 4)[-1]
+    assert r["n_evals"] == n4
+    assert abs(r["integral"] - I4) < 1e-13 * abs(I4)
+
+
+def test_probe_refinement_beats_uniform_on_a_localized_peak():
+    """z = 2.002: a peak of width ~0.06 at k = 0.  Uniform refinement from 16 x 16
+    needs 256 x 256 points for 1e-6; the probe proposal refines around the peak
+    only and, probe points included, uses fewer than half as many."""
+    z = 2.002
+    ev = lgf_integrand(z)
+    hist = BM.uniform_levels(ev, 16, 5)
+    need = next(n for I, n in hist if abs(I - lgf(z)) <= 1e-6 * lgf(z))
+    assert need == 256 ** 2
+    m = BM.WindowedMesh(ev, n0=16, lmax=6, im_e=0.01)
+    r = m.integrate("probe", theta=1e-8)
+    assert abs(r["integral"] - lgf(z)) <= 1e-6 * lgf(z)
+    assert r["n_evals"] < need / 2
+
+
+def test_a_planted_missing_refinement_is_caught():
+    """Proposing nothing at a level where the peak is unresolved leaves the
+    error of the coarse trapezoid rule."""
+    z = 2.002
+    m = BM.WindowedMesh(lgf_integrand(z), n0=16, lmax=6, im_e=0.01)
+    r = m.integrate("learned", theta=1e-8, predict=lambda x: np.full(len(x), -20.0))
+    assert r["levels"] == 0 and r["n_evals"] == 256
+    assert abs(r["integral"] - lgf(z)) > 1e-3 * lgf(z)
+
+
+def test_cell_features_are_finite_and_level_local():
+    m = BM.WindowedMesh(lgf_integrand(2.01), n0=16, lmax=3, im_e=0.02)
+    m.ensure(0, np.ones((16, 16), bool))
+    x = m.cell_features(0, np.ones((16, 16)), 1.0)
+    assert x.shape == (256, BM.NFEAT) and np.isfinite(x).all()
+    assert m.n_evals == 256                       # features use level-0 points only
+    j = BM.FEATURES.index("log_d2")
+    # the roughness feature peaks at the cells touching k = 0 (index 8 of 16)
+    peak = np.unravel_index(np.argmax(x[:, j]), (16, 16))
+    assert set(peak) <= {7, 8}
+    assert np.allclose(x[:, BM.FEATURES.index("log_imE")], math.log10(0.02))
