@@ -119,7 +119,10 @@ int skkr_tau_bz(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it
  *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 128
  *   "inv_kernel" 0 blocked MFMA Gauss-Jordan | 1 row-wise Gauss-Jordan
- *                (different rounding; not bitwise equal to 0) */
+ *                (different rounding; not bitwise equal to 0)
+ *   "gemm_3m"   0 four-product | 1 three-product (Gauss) complex GEMM on the
+ *               128 tile: 3/4 of the MFMA work, normwise FP64 accuracy,
+ *               different rounding (also SKKR_GEMM_3M=1 in the environment) */
 int skkr_set_option(skkr_ctx *ctx, const char *key, int value);
 
 typedef struct {

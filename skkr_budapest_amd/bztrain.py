@@ -128,10 +128,20 @@ def load_model(path=MODEL_PATH, device: int = 0):
     return m
 
 
-def learned_integral(evaluate, model, n0: int, lmax: int, im_e: float, theta: float,
-                     margin: float = 0.5):
+def integrate_bz(evaluate, n0: int, lmax: int, im_e: float, tol: float = 1e-6, model=None,
+                 theta: float = THETA, margin: float = MARGIN):
+    """BZ average of f with the learned proposal, or -- without a model (none
+    trained, or trained on other features) -- the fallback: plain uniform
+    refinement until two successive levels agree to tol."""
+    if model is None:
+        hist = BM.uniform_levels(evaluate, n0, lmax, tol=tol)
+        converged = len(hist) > 1 and abs(hist[-1][0] - hist[-2][0]) <= tol * abs(hist[-1][0])
+        return {"integral": hist[-1][0], "n_evals": hist[-1][1], "method": "uniform",
+                "converged": converged}
     m = BM.WindowedMesh(evaluate, n0=n0, lmax=lmax, im_e=im_e)
-    return m.integrate("learned", theta=theta, predict=model.predict, margin=margin)
+    r = m.integrate("learned", theta=theta, predict=model.predict, margin=margin)
+    r["method"] = "learned"
+    return r
 
 
 def compare(evaluate, model, n0: int, lmax: int, im_e: float, tol: float = 1e-6,

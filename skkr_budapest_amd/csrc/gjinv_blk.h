@@ -11,9 +11,9 @@
 //   1. block column K -> LDS;
 //   2. wave 0 inverts D in LDS (in-place Gauss-Jordan, partial pivoting within
 //      the 16 rows, then unscramble) -> Dinv;
-//   3. W = -M[O,K] Dinv for the rows O outside block K, W[K] = Dinv;
-//      R = rows of block K in the other block columns;
-//   4. X = Dinv R;
+//   3. W = -M[O,K] Dinv for the rows O outside block K, W[K] = Dinv, and
+//      X = Dinv R for R the rows of block K in the other block columns, as
+//      16x16 MFMA tile products, one tile per wave;
 //   5. rank-16 MFMA update of every other tile: M[O, J] += W[O, :] R;
 //   6. rows of block K <- X; block column K <- W.
 // The matrices inverted here (t^-1 - G(k), the Schur complements and the
@@ -30,7 +30,6 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
   constexpr int WC = T / TJ;
   constexpr int NW = (T / TI) * WC;
   constexpr int NT = NW * 64;
-  constexpr int PER = 16 * N / NT;  // block-column elements per thread
   __shared__ double2 pan[16 * N];   // block column K, [t][r]
   __shared__ double2 Wb[16 * N];    // W, [t][r]
   __shared__ double2 Rb[16 * N];    // R, [t][c]
@@ -92,6 +91,16 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             pan[l16 * N + r] = make_double2(mre[i][j][q], mim[i][j][q]);
           }
         }
+    // R = rows of block K in the other block columns -> Rb[t][c]
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+      if (TI * wr + i == k)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+          if (TJ * wc + j != k)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              Rb[(l4 + 4 * q) * N + (TJ * wc + j) * 16 + l16] = make_double2(mre[i][j][q], mim[i][j][q]);
     __syncthreads();
     // (2) wave 0: Dinv = D^-1, in-place Gauss-Jordan with partial pivoting
     // inside the block.  Lane l holds column l%16, rows l/16 + 4q.
@@ -165,53 +174,51 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
       }
     }
     __syncthreads();
-    // (3) W = -pan[O] Dinv (rows outside block K), W[K] = Dinv; R = rows of K
+    // (3) on MFMA, one 16x16 tile per wave and step:
+    //     W[rho] = -pan[rho] Dinv for row tiles rho != K, W[K] = Dinv;
+    //     X[gam] = Dinv R[gam] for column tiles gam != K.
 #pragma unroll 1
-    for (int m = 0; m < PER; ++m) {
-      const int e = tid + NT * m;
-      const int r = e % N, t = e / N;
-      double2 v;
-      if (r / 16 == k) {
-        v = Dinv[t * 16 + (r - 16 * k)];
-      } else {
-        double2 acc = make_double2(0.0, 0.0);
-#pragma unroll 4
-        for (int u = 0; u < 16; ++u) {
-          const double2 a = pan[u * N + r];
-          const double2 di = Dinv[t * 16 + u];
-          acc.x += a.x * di.x - a.y * di.y;
-          acc.y += a.x * di.y + a.y * di.x;
+    for (int rho = w; rho < T; rho += NW) {
+      if (rho == k) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int u = l4 + 4 * q;
+          Wb[l16 * N + 16 * k + u] = Dinv[l16 * 16 + u];
         }
-        v = make_double2(-acc.x, -acc.y);
+        continue;
       }
-      Wb[t * N + r] = v;
+      dx4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const double2 a = pan[(4 * ks + l4) * N + 16 * rho + l16];
+        const double2 d = Dinv[l16 * 16 + 4 * ks + l4];
+        are = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, d.x, are, 0, 0, 0);
+        are = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, d.y, are, 0, 0, 0);
+        aim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, d.y, aim, 0, 0, 0);
+        aim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, d.x, aim, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        Wb[l16 * N + 16 * rho + l4 + 4 * q] = make_double2(-are[q], -aim[q]);
     }
-#pragma unroll
-    for (int i = 0; i < TI; ++i)
-      if (TI * wr + i == k)
-#pragma unroll
-        for (int j = 0; j < TJ; ++j)
-          if (TJ * wc + j != k)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-              Rb[(l4 + 4 * q) * N + (TJ * wc + j) * 16 + l16] = make_double2(mre[i][j][q], mim[i][j][q]);
-    __syncthreads();
-    // (4) X = Dinv R
 #pragma unroll 1
-    for (int m = 0; m < PER; ++m) {
-      const int e = tid + NT * m;
-      const int t = e / N, c = e % N;
-      if (c / 16 == k) continue;
-      double2 acc = make_double2(0.0, 0.0);
-#pragma unroll 4
-      for (int u = 0; u < 16; ++u) {
-        const double2 di = Dinv[u * 16 + t];
-        const double2 rr = Rb[u * N + c];
-        acc.x += di.x * rr.x - di.y * rr.y;
-        acc.y += di.x * rr.y + di.y * rr.x;
+    for (int gam = w; gam < T; gam += NW) {
+      if (gam == k) continue;
+      dx4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const double2 d = Dinv[(4 * ks + l4) * 16 + l16];
+        const double2 r = Rb[(4 * ks + l4) * N + 16 * gam + l16];
+        are = __builtin_amdgcn_mfma_f64_16x16x4f64(d.x, r.x, are, 0, 0, 0);
+        are = __builtin_amdgcn_mfma_f64_16x16x4f64(-d.y, r.y, are, 0, 0, 0);
+        aim = __builtin_amdgcn_mfma_f64_16x16x4f64(d.x, r.y, aim, 0, 0, 0);
+        aim = __builtin_amdgcn_mfma_f64_16x16x4f64(d.y, r.x, aim, 0, 0, 0);
       }
-      Xb[t * N + c] = acc;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        Xb[(l4 + 4 * q) * N + 16 * gam + l16] = make_double2(are[q], aim[q]);
     }
+    __syncthreads();
     // (5) rank-16 update of the tiles outside block row K and block column K
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
@@ -232,8 +239,8 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             mim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].y, bfr[j].x, mim[i][j], 0, 0, 0);
           }
     }
-    __syncthreads();
-    // (6) rows of block K <- X, block column K <- W
+    // (6) rows of block K <- X, block column K <- W (LDS reads only: no barrier
+    // needed after (5))
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll

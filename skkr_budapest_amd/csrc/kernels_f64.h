@@ -92,18 +92,23 @@ struct EpiArgs {
 // ---------------------------------------------------------------------------
 // Batched complex GEMM on FP64 MFMA:
 //   C[b] = alpha * A[b](m x k) * B[b](k x n) + beta * C[b]
-// Work-group tile TM x TN, KC-deep LDS stages, each wave a 32x32 sub-tile of
-// 2x2 v_mfma_f64_16x16x4 blocks.  The MFMA computes the transposed block
-// (B^T A^T) so that lanes 0..15 own 16 consecutive rows of C and the stores
-// of a column are contiguous.
-// Complex product with two accumulators: re += ar*br - ai*bi, im += ar*bi + ai*br.
+// Work-group tile TM x TN, KC-deep LDS stages, each wave a 32 x 16*WT
+// sub-tile of 2 x WT v_mfma_f64_16x16x4 blocks.  The MFMA computes the
+// transposed block (B^T A^T) so that lanes 0..15 own 16 consecutive rows of C
+// and the stores of a column are contiguous.
+// Complex product, M3 = false: four real products into two accumulators,
+//   re += ar*br - ai*bi, im += ar*bi + ai*br;
+// M3 = true: three real products (Gauss) into three accumulators,
+//   p1 += ar*br, p2 += ai*bi, p3 += (ar+ai)*(br+bi);  re = p1 - p2,
+//   im = p3 - p1 - p2  (3/4 of the MFMA work; normwise FP64 accuracy).
 // ---------------------------------------------------------------------------
-template <int TM, int TN, int KC>
-__global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
+template <int TM, int TN, int KC, int WT = 2, bool M3 = false>
+__global__ __launch_bounds__((TM / 32) * (TN / (16 * WT)) * 64) void zgemm_f64_kernel(
     int m, int n, int k, const double2* const* __restrict__ Ap, int lda,
     const double2* const* __restrict__ Bp, int ldb, double2* const* __restrict__ Cp,
     int ldc, double2 alpha, double2 beta, const int* __restrict__ active, EpiArgs epi) {
-  constexpr int NW = (TM / 32) * (TN / 32);
+  constexpr int WTN = 16 * WT;               // wave tile width
+  constexpr int NW = (TM / 32) * (TN / WTN);
   constexpr int NT = NW * 64;
   // double-buffered LDS stages; the next stage is fetched into registers while
   // the MFMAs of the current one run
@@ -121,14 +126,18 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
   double2* __restrict__ C = Cp ? Cp[b] : nullptr;  // epilogue modes write elsewhere
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w % (TM / 32), wn = w / (TM / 32);
-  dx4 acc_re[2][2], acc_im[2][2];
+  constexpr int P2 = M3 ? WT : 1;
+  dx4 acc_re[2][WT], acc_im[2][WT], acc_p2[2][P2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 2; ++i) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WT; ++j) {
       acc_re[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
       acc_im[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
     }
+#pragma unroll
+    for (int j = 0; j < P2; ++j) acc_p2[i][j] = (dx4){0.0, 0.0, 0.0, 0.0};
+  }
   const int l16 = lane & 15, l4 = lane >> 4;
   double2 ra[LA], rb[LB];
   auto fetch = [&](int k0) {
@@ -170,26 +179,52 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
     if (more) fetch(k0 + KC);
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
-      double2 af[2], bf[2];
+      double2 af[2], bf[WT];
 #pragma unroll
       for (int i = 0; i < 2; ++i) af[i] = As[buf][ks + l4][wm * 32 + i * 16 + l16];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][wn * 32 + j * 16 + l16][ks + l4];
+      for (int j = 0; j < WT; ++j) bf[j] = Bs[buf][wn * WTN + j * 16 + l16][ks + l4];
+      // transposed block: MFMA A-operand = B^T (lane -> col of C),
+      // B-operand = A^T (lane -> row of C)
+      if constexpr (M3) {
+        double as[2], bs[WT];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i) as[i] = af[i].x + af[i].y;
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          // transposed block: MFMA A-operand = B^T (lane -> col of C),
-          // B-operand = A^T (lane -> row of C)
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].x, acc_re[i][j], 0, 0, 0);
-          acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bf[j].y, af[i].y, acc_re[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].y, acc_im[i][j], 0, 0, 0);
-          acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].y, af[i].x, acc_im[i][j], 0, 0, 0);
-        }
+        for (int j = 0; j < WT; ++j) bs[j] = bf[j].x + bf[j].y;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < WT; ++j) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].x, acc_re[i][j], 0, 0, 0);
+            acc_p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].y, af[i].y, acc_p2[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[j], as[i], acc_im[i][j], 0, 0, 0);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < WT; ++j) {
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].x, acc_re[i][j], 0, 0, 0);
+            acc_re[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bf[j].y, af[i].y, acc_re[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].x, af[i].y, acc_im[i][j], 0, 0, 0);
+            acc_im[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j].y, af[i].x, acc_im[i][j], 0, 0, 0);
+          }
+      }
     }
     if (more) stash(buf ^ 1);
     __syncthreads();
     buf ^= 1;
+  }
+  if constexpr (M3) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < WT; ++j) {
+        const dx4 p1 = acc_re[i][j], p2 = acc_p2[i][j];
+        acc_re[i][j] = p1 - p2;
+        acc_im[i][j] = acc_im[i][j] - p1 - p2;
+      }
   }
   // v_mfma_f64_16x16x4 result layout: lane l, register q holds D[l/16 + 4q][l%16];
   // D is the transposed block, so C row (l % 16), C column (l/16 + 4q)
@@ -198,11 +233,11 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < WT; ++j)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           int gr = row0 + wm * 32 + i * 16 + l16;
-          int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
+          int gc = col0 + wn * WTN + j * 16 + l4 + 4 * q;
           if (gr < m && gc < n) {
             double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
             double2 r = cmul(alpha, v);
@@ -221,11 +256,11 @@ __global__ __launch_bounds__((TM / 32) * (TN / 32) * 64) void zgemm_f64_kernel(
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < WT; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int gr = row0 + wm * 32 + i * 16 + l16;
-        const int gc = col0 + wn * 32 + j * 16 + l4 + 4 * q;
+        const int gc = col0 + wn * WTN + j * 16 + l4 + 4 * q;
         if (gr < m && gc < n) {
           const double2 v = make_double2(acc_re[i][j][q], acc_im[i][j][q]);
           if (gr < hn) {

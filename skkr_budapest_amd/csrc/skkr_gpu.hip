@@ -101,6 +101,7 @@ struct skkr_ctx {
   double dom_ms = 0.0, dom_flops = 0.0, dom_bytes = 0.0;
   long long dom_launches = 0;
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
+  int gemm_3m = 0;     // 128-wide tiles: 1 = three-product (Gauss) complex GEMM
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 128
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
   int graphs = 0;
@@ -154,7 +155,9 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_t(c, &c->eps, B2 * nn));
   CHK(dalloc_t(c, &c->xs, B2 * nn));
   CHK(dalloc_t(c, &c->AB, 2 * B2 * 2 * nn));
-  CHK(dalloc_t(c, &c->dnorm, B2 * 64));
+  // |alpha'|^2 partials: one per work-group of the 2n x n product, at most
+  // the count of the smallest (32 x 32) tile
+  CHK(dalloc_t(c, &c->dnorm, (size_t)B2 * ((2 * n + 31) / 32) * ((n + 31) / 32)));
   CHK(dalloc_t(c, &c->Yinv, B2 * nn));
   CHK(dalloc_t(c, &c->AYBY, B2 * 2 * nn));
   CHK(dalloc_t(c, &c->active, B2));
@@ -277,7 +280,9 @@ static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   if (c && c->tile_2nn && m == 2 * c->n && n == c->n) t = c->tile_2nn;
   if (c && c->gemm_tile_force > 0) t = c->gemm_tile_force;
   *tm = t;
-  *tiles = ((m + t - 1) / t) * ((n + t - 1) / t);
+  // the three-product kernel of the 128 tile covers 128 x 64
+  const int tn = (t == 128 && c && c->gemm_3m) ? 64 : t;
+  *tiles = ((m + t - 1) / t) * ((n + tn - 1) / tn);
 }
 
 static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
@@ -291,7 +296,10 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
   gemm_tile(c, m, n, &tm, &tiles);
   dim3 grid(tiles, nb);
   dom_begin(c);
-  if (tm == 128)
+  if (tm == 128 && c && c->gemm_3m)
+    zgemm_f64_kernel<128, 64, 16, 2, true><<<grid, 512, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (tm == 128)
     zgemm_f64_kernel<128, 128, 16><<<grid, 1024, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   else if (tm == 64)
@@ -613,6 +621,7 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
   if (const char* e = getenv("SKKR_INV_KERNEL")) c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : 0;
+  if (const char* e = getenv("SKKR_GEMM_3M")) c->gemm_3m = (atoi(e) != 0) ? 1 : 0;
   if (const char* e = getenv("SKKR_GEMM_TILE")) {
     const int t = atoi(e);
     if (t == 32 || t == 64 || t == 128) c->gemm_tile_force = t;
@@ -715,8 +724,10 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
   if (!c->have_structure || !c->have_tinv) return SKKR_ERR_ARG;
   HIPCHK(hipSetDevice(c->cfg.device));
   auto t0 = std::chrono::steady_clock::now();
-  const int B = auto_batch(c, std::max(nk, 1));
-  CHK(ensure_batch(c, B));
+  CHK(ensure_batch(c, auto_batch(c, std::max(nk, 1))));
+  // the chain layout on the device (left chains 0..B-1, right chains B..2B-1)
+  // follows the allocated batch, which an earlier call may have made larger
+  const int B = c->B;
   const int P = c->cfg.n_pl, lk = c->cfg.lk, ns = c->ns;
   const size_t sites = (size_t)P * ns, tsz = sites * lk * lk;
   c->timing_detail = st ? st->timing_detail : 0;
@@ -831,6 +842,12 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
   if (strcmp(key, "inv_kernel") == 0) {
     if (value != 0 && value != 1) return SKKR_ERR_ARG;
     c->inv_kernel = value;
+    drop_graphs(c);
+    return SKKR_OK;
+  }
+  if (strcmp(key, "gemm_3m") == 0) {
+    if (value != 0 && value != 1) return SKKR_ERR_ARG;
+    c->gemm_3m = value;
     drop_graphs(c);
     return SKKR_OK;
   }

@@ -48,6 +48,23 @@ def test_zgemm_matches_numpy(gpu, n):
     assert rel(c, ref) < 1e-14
 
 
+@pytest.mark.parametrize("n", [64, 128, 200])
+def test_zgemm_three_product_matches_numpy(gpu, n):
+    """The Gauss three-product complex GEMM (option gemm_3m, 128 x 64 tiles):
+    normwise FP64 accuracy, ragged edges included (n = 200)."""
+    rng = np.random.default_rng(300 + n)
+    a = rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))
+    b = rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        ctx.set_option("gemm_3m", 1)
+        c3 = ctx.zgemm(a, b)
+        ctx.set_option("gemm_3m", 0)
+        c4 = ctx.zgemm(a, b)
+    ref = a @ b
+    assert rel(c3, ref) < 1e-14
+    assert rel(c4, ref) < 1e-14
+
+
 @pytest.mark.parametrize("kernel", ["blocked", "rowwise"])
 @pytest.mark.parametrize("n", [1, 9, 18, 36, 64, 100, 128])
 def test_gjinv_matches_numpy(gpu, n, kernel, monkeypatch):
@@ -184,6 +201,23 @@ def test_config1_sample_matches_oracle(gpu):
     assert np.abs(r["trk"] - ref["trk"]).max() < TOL_CFG1 * np.abs(ref["trk"]).max()
 
 
+def test_config1_sample_with_three_product_gemm(gpu):
+    """Config 1 with the three-product GEMM in the decimation and sweeps."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.array([0, 1300, 2048, 3500])
+    ks, ws = k[idx], w[idx]
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=8)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_option("gemm_3m", 1)
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, ks, ws)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL_CFG1
+
+
 def test_config1_full_mesh_properties(gpu):
     """Full config-1 energy (4096 k): per-k results do not depend on how the k
     are batched (bitwise), and the BZ sum is linear in the weights."""
@@ -282,6 +316,25 @@ def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
         assert t["tile_nn"] in (32, 64, 128) and t["tile_2nn"] in (32, 64, 128)
         r = ctx.tau_bz(E, k, w)
         np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+
+
+def test_smaller_call_after_a_larger_batch(gpu):
+    """A call with fewer k-points than the batch an earlier call allocated
+    (the adaptive mesh does this level by level) gives the same bits as a
+    fresh context."""
+    sy = small_sys(lk=8, s=2, P=4, r2max=4, E=0.35 + 0.03j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(12)                      # 144 k
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ctx.tau_bz(E, k, w)                         # allocates a batch of 144
+        small = ctx.tau_bz(E, k[5:22], w[5:22])     # 17 k
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        fresh = ctx.tau_bz(E, k[5:22], w[5:22])
+    np.testing.assert_array_equal(small["trk"], fresh["trk"])
+    np.testing.assert_array_equal(small["iters"], fresh["iters"])
+    np.testing.assert_array_equal(small["tau_bz"], fresh["tau_bz"])
 
 
 def test_set_option_rejects_unknown(gpu):
