@@ -24,18 +24,112 @@
 #pragma once
 #include "kernels_f64.h"
 
+// 16x16 Gauss-Jordan inversion of a diagonal block held by one wave in the
+// v_mfma_f64_16x16x4 result layout (lane l, element q: row l/16 + 4q, column
+// l%16), with partial pivoting within the 16 rows.  The pivot search, the
+// pivot row and the pivot column move between lanes by shuffles.  Writes the
+// unscrambled inverse to dinv[col * DP + row]; sets *sing on a zero or NaN
+// pivot.
+__device__ __forceinline__ double2 shfl2(double2 v, int src) {
+  return make_double2(__shfl(v.x, src), __shfl(v.y, src));
+}
+// value of lane `src` (wave-uniform) in every lane, without an LDS round trip
+__device__ __forceinline__ double readlane_f64(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int DP>
+__device__ __forceinline__ void diag_inverse16(double2 d[4], int lane, double2* __restrict__ dinv,
+                                               int* sperm, int* spinv, int* sing) {
+  const int l16 = lane & 15, l4 = lane >> 4;
+  unsigned used = 0;  // rows already pivoted (identical in every lane)
+  bool bad = false;
+#pragma unroll 1
+  for (int it = 0; it < 16; ++it) {
+    // candidates of column it: rows l4 + 4q of lane l4*16 + it
+    double best = -1.0;
+    int p = 16;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = l4 + 4 * q;
+      const double mg = d[q].x * d[q].x + d[q].y * d[q].y;
+      if (!((used >> r) & 1u) && mg > best) { best = mg; p = r; }
+    }
+    // largest over the four lanes of column it (read lane by lane, so the
+    // choice is wave-uniform); the first row wins ties
+    double bb = readlane_f64(best, it);
+    int pb = __builtin_amdgcn_readlane(p, it);
+#pragma unroll
+    for (int m = 1; m < 4; ++m) {
+      const double ob = readlane_f64(best, it + 16 * m);
+      const int op = __builtin_amdgcn_readlane(p, it + 16 * m);
+      if (ob > bb || (ob == bb && op < pb)) { bb = ob; pb = op; }
+    }
+    p = pb;
+    if (p > 15) {  // only with NaN entries: take the first unused row
+      p = __builtin_ctz(~used & 0xFFFFu);
+      bad = true;
+    }
+    if (!(bb > 0.0)) bad = true;
+    used |= 1u << p;
+    if (lane == 0) {
+      sperm[it] = p;
+      spinv[p] = it;
+    }
+    // pivot row p of this lane's column: lane (p&3)*16 + l16 holds it in
+    // element p>>2 (all four elements are moved: an index into d by the
+    // wave-uniform p>>2 would be lowered to scratch memory)
+    const int qp = p >> 2, src = (p & 3) * 16 + l16;
+    const double2 t0 = shfl2(d[0], src), t1 = shfl2(d[1], src);
+    const double2 t2 = shfl2(d[2], src), t3 = shfl2(d[3], src);
+    const double2 prow = qp == 0 ? t0 : qp == 1 ? t1 : qp == 2 ? t2 : t3;
+    // column it of this lane's rows
+    double2 f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f[q] = shfl2(d[q], l4 * 16 + it);
+    const double2 piv = make_double2(readlane_f64(prow.x, it), readlane_f64(prow.y, it));
+    const double2 qv = crecip(piv);
+    const double2 rn = (l16 == it) ? qv : cmul(prow, qv);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = l4 + 4 * q;
+      if (r == p) {
+        d[q] = rn;
+      } else {
+        double2 o = (l16 == it) ? make_double2(0.0, 0.0) : d[q];
+        o.x -= f[q].x * rn.x - f[q].y * rn.y;
+        o.y -= f[q].x * rn.y + f[q].y * rn.x;
+        d[q] = o;
+      }
+    }
+  }
+  if (bad && lane == 0) *sing = 1;
+  // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int cc = sperm[l16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dinv[cc * DP + spinv[l4 + 4 * q]] = d[q];
+}
+
 template <int T, int TI, int TJ>
 __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel(InvArgs args) {
   constexpr int N = 16 * T;
   constexpr int WC = T / TJ;
   constexpr int NW = (T / TI) * WC;
   constexpr int NT = NW * 64;
-  __shared__ double2 pan[16 * N];   // block column K, [t][r]
-  __shared__ double2 Wb[16 * N];    // W, [t][r]
+  // [t][r] arrays padded to a row stride of N+1 (and 16x16 blocks to 17) so
+  // that the 16 lanes of a column-strided access hit different LDS banks
+  constexpr int NP = N + 1, DP = 17;
+  __shared__ double2 pan[16 * NP];  // block column K, [t][r]
+  __shared__ double2 Wb[16 * NP];   // W, [t][r]
   __shared__ double2 Rb[16 * N];    // R, [t][c]
   __shared__ double2 Xb[16 * N];    // X, [t][c]
-  __shared__ double2 Db[16 * 16];   // D work area, [col][row]
-  __shared__ double2 Dinv[16 * 16]; // [col][row]
+  __shared__ double2 Dinv[16 * DP]; // [col][row]
+  __shared__ int sperm[16], spinv[16];  // pivot rows of the diagonal block
   __shared__ int s_sing;
   __shared__ double2 diag[N];
 
@@ -77,21 +171,30 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
   }
   if (tid == 0) s_sing = 0;
 
-#pragma unroll 1
-  for (int k = 0; k < T; ++k) {
-    // (1) block column K -> pan[t][r]
+  // prologue: block column 0 -> pan; the owner of tile (0,0) inverts D_0
+  {
+    double2 d[4];
+    bool own = false;
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
       for (int j = 0; j < TJ; ++j)
-        if (TJ * wc + j == k) {
+        if (TJ * wc + j == 0) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = (TI * wr + i) * 16 + l4 + 4 * q;
-            pan[l16 * N + r] = make_double2(mre[i][j][q], mim[i][j][q]);
+          for (int q = 0; q < 4; ++q)
+            pan[l16 * NP + (TI * wr + i) * 16 + l4 + 4 * q] = make_double2(mre[i][j][q], mim[i][j][q]);
+          if (TI * wr + i == 0) {
+            own = true;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[q] = make_double2(mre[i][j][q], mim[i][j][q]);
           }
         }
-    // R = rows of block K in the other block columns -> Rb[t][c]
+    if (own) diag_inverse16<DP>(d, lane, Dinv, sperm, spinv, &s_sing);
+  }
+
+#pragma unroll 1
+  for (int k = 0; k < T; ++k) {
+    // (1) R = rows of block K in the other block columns -> Rb[t][c]
 #pragma unroll
     for (int i = 0; i < TI; ++i)
       if (TI * wr + i == k)
@@ -101,80 +204,8 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
 #pragma unroll
             for (int q = 0; q < 4; ++q)
               Rb[(l4 + 4 * q) * N + (TJ * wc + j) * 16 + l16] = make_double2(mre[i][j][q], mim[i][j][q]);
-    __syncthreads();
-    // (2) wave 0: Dinv = D^-1, in-place Gauss-Jordan with partial pivoting
-    // inside the block.  Lane l holds column l%16, rows l/16 + 4q.
-    if (w == 0) {
-      double2 d[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) d[q] = pan[l16 * N + 16 * k + l4 + 4 * q];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) Db[l16 * 16 + l4 + 4 * q] = d[q];
-      __builtin_amdgcn_wave_barrier();
-      unsigned used = 0;  // rows already pivoted (identical in every lane)
-      int perm[16];
-#pragma unroll
-      for (int it = 0; it < 16; ++it) {
-        int p = 0;
-        double best = -1.0;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const double2 v = Db[it * 16 + r];
-          const double mg = v.x * v.x + v.y * v.y;
-          if (!((used >> r) & 1u) && mg > best) { best = mg; p = r; }
-        }
-        if ((used >> p) & 1u) {  // only with NaN entries: take the first unused row
-          p = __builtin_ctz(~used & 0xFFFFu);
-          if (lane == 0) s_sing = 1;
-        }
-        used |= 1u << p;
-        perm[it] = p;
-        const double2 piv = Db[it * 16 + p];
-        if (best <= 0.0 && lane == 0) s_sing = 1;
-        const double2 qv = crecip(piv);
-        const double2 prow = Db[l16 * 16 + p];
-        const double2 rn = (l16 == it) ? qv : cmul(prow, qv);
-        double2 f[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) f[q] = Db[it * 16 + l4 + 4 * q];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int r = l4 + 4 * q;
-          if (r == p) {
-            d[q] = rn;
-          } else {
-            double2 o = (l16 == it) ? make_double2(0.0, 0.0) : d[q];
-            o.x -= f[q].x * rn.x - f[q].y * rn.y;
-            o.y -= f[q].x * rn.y + f[q].y * rn.x;
-            d[q] = o;
-          }
-        }
-        // every lane has read this step's values before any lane writes
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Db[l16 * 16 + l4 + 4 * q] = d[q];
-        __builtin_amdgcn_wave_barrier();
-      }
-      // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
-      int pinv[16];
-#pragma unroll
-      for (int t = 0; t < 16; ++t) pinv[t] = 0;
-#pragma unroll
-      for (int t = 0; t < 16; ++t) pinv[perm[t] & 15] = t;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rho = l4 + 4 * q, gam = l16;
-        int rr = 0, cc = 0;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          if (t == rho) rr = pinv[t];
-          if (t == gam) cc = perm[t];
-        }
-        Dinv[cc * 16 + rr] = d[q];
-      }
-    }
-    __syncthreads();
-    // (3) on MFMA, one 16x16 tile per wave and step:
+    __syncthreads();  // pan (column K), Dinv and Rb are complete
+    // (2) on MFMA, one 16x16 tile per wave and step:
     //     W[rho] = -pan[rho] Dinv for row tiles rho != K, W[K] = Dinv;
     //     X[gam] = Dinv R[gam] for column tiles gam != K.
 #pragma unroll 1
@@ -183,15 +214,15 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int u = l4 + 4 * q;
-          Wb[l16 * N + 16 * k + u] = Dinv[l16 * 16 + u];
+          Wb[l16 * NP + 16 * k + u] = Dinv[l16 * DP + u];
         }
         continue;
       }
       dx4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const double2 a = pan[(4 * ks + l4) * N + 16 * rho + l16];
-        const double2 d = Dinv[l16 * 16 + 4 * ks + l4];
+        const double2 a = pan[(4 * ks + l4) * NP + 16 * rho + l16];
+        const double2 d = Dinv[l16 * DP + 4 * ks + l4];
         are = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, d.x, are, 0, 0, 0);
         are = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, d.y, are, 0, 0, 0);
         aim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, d.y, aim, 0, 0, 0);
@@ -199,7 +230,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        Wb[l16 * N + 16 * rho + l4 + 4 * q] = make_double2(-are[q], -aim[q]);
+        Wb[l16 * NP + 16 * rho + l4 + 4 * q] = make_double2(-are[q], -aim[q]);
     }
 #pragma unroll 1
     for (int gam = w; gam < T; gam += NW) {
@@ -207,7 +238,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
       dx4 are = {0.0, 0.0, 0.0, 0.0}, aim = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const double2 d = Dinv[(4 * ks + l4) * 16 + l16];
+        const double2 d = Dinv[(4 * ks + l4) * DP + l16];
         const double2 r = Rb[(4 * ks + l4) * N + 16 * gam + l16];
         are = __builtin_amdgcn_mfma_f64_16x16x4f64(d.x, r.x, are, 0, 0, 0);
         are = __builtin_amdgcn_mfma_f64_16x16x4f64(-d.y, r.y, are, 0, 0, 0);
@@ -218,48 +249,99 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
       for (int q = 0; q < 4; ++q)
         Xb[(l4 + 4 * q) * N + 16 * gam + l16] = make_double2(are[q], aim[q]);
     }
-    __syncthreads();
-    // (5) rank-16 update of the tiles outside block row K and block column K
+    __syncthreads();  // W and X are complete; pan (column K) and Dinv are free
+    // (3) rank-16 update M[O,J] += W[O] R[J] of the tiles outside block row K
+    // and block column K, in two passes: first block column K+1 (look ahead:
+    // it goes to pan, and the owner of tile (K+1, K+1) inverts D_{K+1} while
+    // the other waves run the second pass), then the rest
+#pragma unroll 1
+    for (int ks = 0; ks < (k + 1 < T ? 4 : 0); ++ks) {
+      const int t = 4 * ks + l4;
+      const double2 bk = Rb[t * N + (k + 1) * 16 + l16];
+#pragma unroll
+      for (int i = 0; i < TI; ++i) {
+        const double2 af = Wb[t * NP + (TI * wr + i) * 16 + l16];
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          const int bi = TI * wr + i, bj = TJ * wc + j;
+          if (bj == k + 1 && bi != k) {
+            mre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af.x, bk.x, mre[i][j], 0, 0, 0);
+            mre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af.y, bk.y, mre[i][j], 0, 0, 0);
+            mim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af.x, bk.y, mim[i][j], 0, 0, 0);
+            mim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af.y, bk.x, mim[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    {
+      // block column K+1 (row block K from X) -> pan; D_{K+1} -> Dinv
+      double2 d[4];
+      bool own = false;
+#pragma unroll
+      for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+          const int bi = TI * wr + i, bj = TJ * wc + j;
+          if (bj == k + 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (bi == k) {
+                const double2 v = Xb[(l4 + 4 * q) * N + bj * 16 + l16];
+                mre[i][j][q] = v.x;
+                mim[i][j][q] = v.y;
+              }
+              pan[l16 * NP + bi * 16 + l4 + 4 * q] = make_double2(mre[i][j][q], mim[i][j][q]);
+            }
+            if (bi == k + 1) {
+              own = true;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) d[q] = make_double2(mre[i][j][q], mim[i][j][q]);
+            }
+          }
+        }
+      if (own) diag_inverse16<DP>(d, lane, Dinv, sperm, spinv, &s_sing);
+    }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
       const int t = 4 * ks + l4;
       double2 af[TI], bfr[TJ];
 #pragma unroll
-      for (int i = 0; i < TI; ++i) af[i] = Wb[t * N + (TI * wr + i) * 16 + l16];
+      for (int i = 0; i < TI; ++i) af[i] = Wb[t * NP + (TI * wr + i) * 16 + l16];
 #pragma unroll
       for (int j = 0; j < TJ; ++j) bfr[j] = Rb[t * N + (TJ * wc + j) * 16 + l16];
 #pragma unroll
       for (int i = 0; i < TI; ++i)
 #pragma unroll
-        for (int j = 0; j < TJ; ++j)
-          if (TJ * wc + j != k && TI * wr + i != k) {
+        for (int j = 0; j < TJ; ++j) {
+          const int bi = TI * wr + i, bj = TJ * wc + j;
+          if (bj != k && bi != k && bj != k + 1) {
             mre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].x, bfr[j].x, mre[i][j], 0, 0, 0);
             mre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-af[i].y, bfr[j].y, mre[i][j], 0, 0, 0);
             mim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].x, bfr[j].y, mim[i][j], 0, 0, 0);
             mim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].y, bfr[j].x, mim[i][j], 0, 0, 0);
           }
+        }
     }
-    // (6) rows of block K <- X, block column K <- W (LDS reads only: no barrier
-    // needed after (5))
+    // (4) rows of block K <- X (block column K+1 done above), block column K <- W
 #pragma unroll
     for (int i = 0; i < TI; ++i)
 #pragma unroll
-      for (int j = 0; j < TJ; ++j)
+      for (int j = 0; j < TJ; ++j) {
+        const int bi = TI * wr + i, bj = TJ * wc + j;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = (TI * wr + i) * 16 + l4 + 4 * q;
-          const int c = (TJ * wc + j) * 16 + l16;
-          if (TJ * wc + j == k) {
-            const double2 v = Wb[l16 * N + r];
+          if (bj == k) {
+            const double2 v = Wb[l16 * NP + bi * 16 + l4 + 4 * q];
             mre[i][j][q] = v.x;
             mim[i][j][q] = v.y;
-          } else if (TI * wr + i == k) {
-            const double2 v = Xb[(l4 + 4 * q) * N + c];
+          } else if (bi == k && bj != k + 1) {
+            const double2 v = Xb[(l4 + 4 * q) * N + bj * 16 + l16];
             mre[i][j][q] = v.x;
             mim[i][j][q] = v.y;
           }
         }
-    __syncthreads();
+      }
+    __syncthreads();  // the next pan, Dinv and this step's W, X, R reads are done
   }
 
   if (s_sing && args.status && tid == 0) atomicOr(&args.status[b], 1);

@@ -280,8 +280,8 @@ static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   if (c && c->tile_2nn && m == 2 * c->n && n == c->n) t = c->tile_2nn;
   if (c && c->gemm_tile_force > 0) t = c->gemm_tile_force;
   *tm = t;
-  // the three-product kernel of the 128 tile covers 128 x 64
-  const int tn = (t == 128 && c && c->gemm_3m) ? 64 : t;
+  // the kernels of the 128 tile cover 128 x 64
+  const int tn = (t == 128) ? 64 : t;
   *tiles = ((m + t - 1) / t) * ((n + tn - 1) / tn);
 }
 
@@ -296,14 +296,21 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
   gemm_tile(c, m, n, &tm, &tiles);
   dim3 grid(tiles, nb);
   dom_begin(c);
-  if (tm == 128 && c && c->gemm_3m)
+  const bool m3 = c && c->gemm_3m;
+  if (tm == 128 && m3)
     zgemm_f64_kernel<128, 64, 16, 2, true><<<grid, 512, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
-  else if (tm == 128)
-    zgemm_f64_kernel<128, 128, 16><<<grid, 1024, 0, c->stream>>>(
+  else if (tm == 128)  // 8-deep stages: 54 KB LDS, two work-groups per CU
+    zgemm_f64_kernel<128, 64, 8><<<grid, 512, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (tm == 64 && m3)
+    zgemm_f64_kernel<64, 64, 16, 2, true><<<grid, 256, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   else if (tm == 64)
     zgemm_f64_kernel<64, 64, 16><<<grid, 256, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (m3)
+    zgemm_f64_kernel<32, 32, 16, 2, true><<<grid, 64, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   else
     zgemm_f64_kernel<32, 32, 16><<<grid, 64, 0, c->stream>>>(
