@@ -41,22 +41,60 @@ __device__ __forceinline__ double readlane_f64(double v, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// 1/z for a pivot z: conj(z) / |z|^2 with the reciprocal of |z|^2 from
+// v_rcp_f64 refined by two Newton-Raphson steps r <- r (2 - s r).  A zero or
+// NaN pivot gives non-finite values and is flagged by the caller.
+__device__ __forceinline__ double2 pivot_recip(double2 z) {
+  const double s2 = z.x * z.x + z.y * z.y;
+  double r = __builtin_amdgcn_rcp(s2);
+  r = fma(r, fma(-s2, r, 1.0), r);
+  r = fma(r, fma(-s2, r, 1.0), r);
+  return make_double2(z.x * r, -z.y * r);
+}
+
+// One elimination step on element q of a lane: row l4 + 4q of column l16.
+__device__ __forceinline__ void gj_update(double2& dq, int r, int p, bool pivcol, double2 fq,
+                                          double2 rn) {
+  if (r == p) {
+    dq = rn;
+  } else {
+    double2 o = pivcol ? make_double2(0.0, 0.0) : dq;
+    o.x -= fq.x * rn.x - fq.y * rn.y;
+    o.y -= fq.x * rn.y + fq.y * rn.x;
+    dq = o;
+  }
+}
+
+// 16x16 Gauss-Jordan inversion of a diagonal block held by one wave in the
+// v_mfma_f64_16x16x4 result layout (lane l, element q: row l/16 + 4q, column
+// l%16), with partial pivoting within the 16 rows.  The elements are four
+// separate variables (an index into an array by the wave-uniform pivot row
+// would be lowered to scratch memory).  Pivot search by lane reads, pivot
+// row and column by shuffles.  Writes the unscrambled inverse to
+// dinv[col * DP + row]; sets *sing on a zero or NaN pivot.
 template <int DP>
-__device__ __forceinline__ void diag_inverse16(double2 d[4], int lane, double2* __restrict__ dinv,
-                                               int* sperm, int* spinv, int* sing) {
+__device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2& d2, double2& d3,
+                                               int lane, double2* __restrict__ dinv, int* sperm,
+                                               int* spinv, int* sing) {
   const int l16 = lane & 15, l4 = lane >> 4;
   unsigned used = 0;  // rows already pivoted (identical in every lane)
   bool bad = false;
 #pragma unroll 1
   for (int it = 0; it < 16; ++it) {
+    // column it of this lane's rows (independent of the pivot: issued first)
+    const int csrc = l4 * 16 + it;
+    const double2 f0 = shfl2(d0, csrc), f1 = shfl2(d1, csrc);
+    const double2 f2 = shfl2(d2, csrc), f3 = shfl2(d3, csrc);
     // candidates of column it: rows l4 + 4q of lane l4*16 + it
     double best = -1.0;
     int p = 16;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = l4 + 4 * q;
-      const double mg = d[q].x * d[q].x + d[q].y * d[q].y;
-      if (!((used >> r) & 1u) && mg > best) { best = mg; p = r; }
+    {
+      const double m0 = d0.x * d0.x + d0.y * d0.y, m1 = d1.x * d1.x + d1.y * d1.y;
+      const double m2 = d2.x * d2.x + d2.y * d2.y, m3 = d3.x * d3.x + d3.y * d3.y;
+      if (!((used >> l4) & 1u) && m0 > best) { best = m0; p = l4; }
+      if (!((used >> (l4 + 4)) & 1u) && m1 > best) { best = m1; p = l4 + 4; }
+      if (!((used >> (l4 + 8)) & 1u) && m2 > best) { best = m2; p = l4 + 8; }
+      if (!((used >> (l4 + 12)) & 1u) && m3 > best) { best = m3; p = l4 + 12; }
     }
     // largest over the four lanes of column it (read lane by lane, so the
     // choice is wave-uniform); the first row wins ties
@@ -79,40 +117,28 @@ __device__ __forceinline__ void diag_inverse16(double2 d[4], int lane, double2* 
       sperm[it] = p;
       spinv[p] = it;
     }
-    // pivot row p of this lane's column: lane (p&3)*16 + l16 holds it in
-    // element p>>2 (all four elements are moved: an index into d by the
-    // wave-uniform p>>2 would be lowered to scratch memory)
-    const int qp = p >> 2, src = (p & 3) * 16 + l16;
-    const double2 t0 = shfl2(d[0], src), t1 = shfl2(d[1], src);
-    const double2 t2 = shfl2(d[2], src), t3 = shfl2(d[3], src);
-    const double2 prow = qp == 0 ? t0 : qp == 1 ? t1 : qp == 2 ? t2 : t3;
-    // column it of this lane's rows
-    double2 f[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) f[q] = shfl2(d[q], l4 * 16 + it);
+    // pivot row p of this lane's column: lane (p&3)*16 + l16, element p>>2
+    const int qp = p >> 2;
+    const double2 mine = qp == 0 ? d0 : qp == 1 ? d1 : qp == 2 ? d2 : d3;
+    const double2 prow = shfl2(mine, (p & 3) * 16 + l16);
     const double2 piv = make_double2(readlane_f64(prow.x, it), readlane_f64(prow.y, it));
-    const double2 qv = crecip(piv);
-    const double2 rn = (l16 == it) ? qv : cmul(prow, qv);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = l4 + 4 * q;
-      if (r == p) {
-        d[q] = rn;
-      } else {
-        double2 o = (l16 == it) ? make_double2(0.0, 0.0) : d[q];
-        o.x -= f[q].x * rn.x - f[q].y * rn.y;
-        o.y -= f[q].x * rn.y + f[q].y * rn.x;
-        d[q] = o;
-      }
-    }
+    const double2 qv = pivot_recip(piv);
+    const bool pivcol = (l16 == it);
+    const double2 rn = pivcol ? qv : cmul(prow, qv);
+    gj_update(d0, l4, p, pivcol, f0, rn);
+    gj_update(d1, l4 + 4, p, pivcol, f1, rn);
+    gj_update(d2, l4 + 8, p, pivcol, f2, rn);
+    gj_update(d3, l4 + 12, p, pivcol, f3, rn);
   }
   if (bad && lane == 0) *sing = 1;
   // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   const int cc = sperm[l16];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) dinv[cc * DP + spinv[l4 + 4 * q]] = d[q];
+  dinv[cc * DP + spinv[l4]] = d0;
+  dinv[cc * DP + spinv[l4 + 4]] = d1;
+  dinv[cc * DP + spinv[l4 + 8]] = d2;
+  dinv[cc * DP + spinv[l4 + 12]] = d3;
 }
 
 template <int T, int TI, int TJ>
@@ -189,7 +215,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             for (int q = 0; q < 4; ++q) d[q] = make_double2(mre[i][j][q], mim[i][j][q]);
           }
         }
-    if (own) diag_inverse16<DP>(d, lane, Dinv, sperm, spinv, &s_sing);
+    if (own) diag_inverse16<DP>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
   }
 
 #pragma unroll 1
@@ -299,7 +325,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             }
           }
         }
-      if (own) diag_inverse16<DP>(d, lane, Dinv, sperm, spinv, &s_sing);
+      if (own) diag_inverse16<DP>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
