@@ -116,11 +116,24 @@ __global__ __launch_bounds__((TM / 32) * (TN / (16 * WT)) * 64) void zgemm_f64_k
   __shared__ double2 Bs[2][TN][KC + 1];
   constexpr int LA = (TM * KC + NT - 1) / NT;  // A elements per thread per stage
   constexpr int LB = (TN * KC + NT - 1) / NT;
-  const int b = blockIdx.y;
+  // Work-groups go to the 8 XCDs round-robin by linear id.  Renumber them so
+  // that the tiles of one product land on one XCD and share its L2 (A is
+  // read by every column tile): linear id L = 8*tiles*g + 8*t + r is tile t
+  // of product 8g + r.  The last nb % 8 products keep the plain numbering.
+  int b = blockIdx.y, tile = blockIdx.x;
+  {
+    const int tiles = gridDim.x;
+    const int L = blockIdx.x + tiles * blockIdx.y;
+    if (tiles > 1 && L < 8 * tiles * (int)(gridDim.y / 8)) {
+      const int g = L / (8 * tiles), rem = L % (8 * tiles);
+      tile = rem / 8;
+      b = 8 * g + rem % 8;
+    }
+  }
   if (active && !active[b]) return;
   const int tiles_m = (m + TM - 1) / TM;
-  const int row0 = (blockIdx.x % tiles_m) * TM;
-  const int col0 = (blockIdx.x / tiles_m) * TN;
+  const int row0 = (tile % tiles_m) * TM;
+  const int col0 = (tile / tiles_m) * TN;
   const double2* __restrict__ A = Ap[b];
   const double2* __restrict__ B = Bp[b];
   double2* __restrict__ C = Cp ? Cp[b] : nullptr;  // epilogue modes write elsewhere
@@ -290,7 +303,7 @@ __global__ __launch_bounds__((TM / 32) * (TN / (16 * WT)) * 64) void zgemm_f64_k
       if (tid < st) red[tid] += red[tid + st];
       __syncthreads();
     }
-    if (tid == 0) epi.norm[(long long)b * epi.tiles + blockIdx.x] = red[0];
+    if (tid == 0) epi.norm[(long long)b * epi.tiles + tile] = red[0];
   }
 }
 
