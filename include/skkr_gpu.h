@@ -114,6 +114,29 @@ int skkr_tau_bz(skkr_ctx *ctx, double e_re, double e_im, int nk,
                 double *tr_tau_k, int *iters_k, int *status_k,
                 skkr_stats *stats);
 
+/* Screening parameters of the interface sites for the physical (unscreened)
+ * representation of tau, phystau.f (skkr_kpar_v6/nrsp-scf/phystau.f:2-58,
+ * rsp-scf/phystau.f, skkr_dbogo_bsf/rsp-scf/phystau_d.f):
+ *   phys = (1 + A t^-1) tau (1 + t^-1 A) - (A + A t^-1 A),  A = diag(a).
+ * a: sites x lk complex (site-major), a[site][i] = alphaintkkr(l(i), site)
+ * with the reference's index map (lml for lm, ldex for kappa-mu, the hole
+ * alphakkrh for the hole half of a BdG block).  l_of_index: lk ints in
+ * [0, nl), the trace group (l) of each basis index.  lk <= 64. */
+int skkr_set_screening(skkr_ctx *ctx, int nl, const int *l_of_index, const double *a);
+
+/* skkr_tau_bz plus the physical representation of the k-loop outputs of
+ * cpacoord (nrsp-scf/cpacoord.f:294-311, then :375):
+ *   tau_bz_phys (sites x lk x lk, accumulated like tau_bz) += L S R - wsum C,
+ *     S = this call's BZ sum, wsum = sum of its weights (the reference's
+ *     phystau of the whole BZ sum when the weights of all calls add to 1);
+ *   taukdiag (nk x nl x sites complex, k fastest: taukdiag(ik,l,li)) = per-l
+ *     traces of phystau(tau(k)) of each site (optional, NULL to skip).
+ * Needs skkr_set_screening. */
+int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
+                     const double *k, const double *w, double *tau_bz,
+                     double *tau_bz_phys, double *taukdiag, int *iters_k,
+                     int *status_k, skkr_stats *stats);
+
 /* Options of a context (no reference counterpart; execution choices that
  * leave FP64 results bitwise unchanged):
  *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it

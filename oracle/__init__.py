@@ -51,6 +51,8 @@ def lib():
         L.orc_pl_blocks.argtypes = [P, P, i, P, P, P]
         L.orc_tau_bz.argtypes = [i, i, i, i, P, P, P, P, P, P, i, d, i, P, P, i,
                                  P, P, P, P, P]
+        L.orc_phystau.argtypes = [i, P, P, P, P, i, i]
+        L.orc_phystau.restype = None
         for f in (L.orc_gjinv, L.orc_surfgfd, L.orc_dugo, L.orc_tau2d, L.orc_tau_bz):
             f.restype = ctypes.c_int
         _LIB = L
@@ -74,6 +76,27 @@ def gjinv(a):
     b = _cm(a)
     st = lib().orc_gjinv(_p(b), a.shape[0], a.shape[0])
     return _uncm(b), st
+
+
+def phystau(tau, tinv1, tinv2, a, idiag=1):
+    """Screened -> physical tau of one site (phystau.f); a: per-index
+    screening parameters (n,).  Returns the new (n, n) matrix."""
+    n = tau.shape[0]
+    t = _cm(tau)
+    t1, t2 = _cm(tinv1), _cm(tinv2)
+    av = np.ascontiguousarray(a, np.complex128)
+    lib().orc_phystau(n, _p(av), _p(t1), _p(t2), _p(t), int(idiag), n)
+    return _uncm(t)
+
+
+def phys_traces(tau, tinv, a, l_of_index, nl):
+    """Per-l traces of phystau(tau) (cpacoord.f:300-309 taukdiag)."""
+    tp = phystau(tau, tinv, tinv, a, 1)
+    d = np.diagonal(tp)
+    out = np.zeros(nl, np.complex128)
+    for i, l in enumerate(l_of_index):
+        out[l] += d[i]
+    return out
 
 
 def surfgfd(ml, m0, mr, itermax=50, errmax=1e-8):

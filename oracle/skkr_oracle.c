@@ -14,6 +14,8 @@
  *   orc_surfgfd  skkr_kpar_v6/main/surfgf.f:2-108   decimation
  *   orc_dugo     skkr_kpar_v6/main/dugo.f:9-120     lead self-energy (bulk)
  *   orc_tau2d    skkr_kpar_v6/main/tau2d.f:2-152    Godfrin sweeps
+ *   orc_phystau  skkr_kpar_v6/nrsp-scf/phystau.f:2-58 (and rsp-scf/phystau.f,
+ *                skkr_dbogo_bsf/rsp-scf/phystau_d.f): screened -> physical tau
  *   orc_fourier  G(k) = sum_R G_r(R) exp(-i k.R)    (no reference routine;
  *                pinned by brute force and closed forms in the tests)
  *   orc_tau_bz   k-loop body of skkr_kpar_v6/rsp-scf/cpacoord_ksave.f:188-391
@@ -143,6 +145,44 @@ void orc_tripmt2(const cplx *u, const cplx *b, const cplx *v, cplx *b1, int n1,
       AT(b1, i, j, ld1) = x;
     }
   free(c);
+}
+
+/* phystau.f:30-58 (nrsp-scf; the rsp-scf and dbogo phystau_d.f bodies are the
+ * same with the kappa-mu and electron/hole index maps).  With A = diag(a), a_i
+ * the screening parameter of basis index i (alphakkr(l(i)), the hole one for
+ * the hole half in dbogo):
+ *   lmat = 1 + A tinv1, rmat = 1 + tinv2 A, cmat = A + A tinv1 A  (:33-45)
+ *   tau <- lmat tau rmat (tripmt, :46); idiag=1: tau <- tau - cmat (:48-54). */
+void orc_phystau(int n, const cplx *a, const cplx *tinv1, const cplx *tinv2, cplx *tau,
+                 int idiag, int ld) {
+  cplx *lmat = (cplx *)calloc((size_t)n * n, sizeof(cplx));
+  cplx *rmat = (cplx *)calloc((size_t)n * n, sizeof(cplx));
+  cplx *cmat = (cplx *)calloc((size_t)n * n, sizeof(cplx));
+  for (int l = 0; l < n; ++l) {
+    const cplx al = a[l];
+    for (int lp = 0; lp < n; ++lp) {
+      const cplx alp = a[lp];
+      lmat[l + (size_t)lp * n] = al * AT(tinv1, l, lp, ld);
+      rmat[l + (size_t)lp * n] = AT(tinv2, l, lp, ld) * alp;
+      cmat[l + (size_t)lp * n] = al * AT(tinv1, l, lp, ld) * alp;
+    }
+    lmat[l + (size_t)l * n] = 1.0 + lmat[l + (size_t)l * n];
+    rmat[l + (size_t)l * n] = 1.0 + rmat[l + (size_t)l * n];
+    cmat[l + (size_t)l * n] = al + cmat[l + (size_t)l * n];
+  }
+  /* tripmt with the work matrices at leading dimension n and tau at ld */
+  cplx *t = (cplx *)malloc(sizeof(cplx) * (size_t)n * n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) t[i + (size_t)j * n] = AT(tau, i, j, ld);
+  orc_tripmt(lmat, t, rmat, n, n, n);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i)
+      AT(tau, i, j, ld) = idiag == 1 ? t[i + (size_t)j * n] - cmat[i + (size_t)j * n]
+                                     : t[i + (size_t)j * n];
+  free(t);
+  free(lmat);
+  free(rmat);
+  free(cmat);
 }
 
 /* surfgf.f:2-108.  Decimation for the surface of a semi-infinite chain whose

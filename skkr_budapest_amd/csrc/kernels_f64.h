@@ -619,3 +619,94 @@ __global__ __launch_bounds__(256) void scale_copy_kernel(
   long long e = blockIdx.x * 256ll + threadIdx.x;
   if (e < ne) dst[e] = make_double2(scale * src[e].x, scale * src[e].y);
 }
+
+// ---------------------------------------------------------------------------
+// Screened -> physical tau of a site block (phystau.f:30-58).  With
+// A = diag(a), a_i the screening parameter of basis index i, and T = t^-1 of
+// the site:
+//   phys = (1 + A T) tau (1 + T A) - (A + A T A).
+// phys_trace_kernel: per (site, k) the sums over basis indices of one l of
+// the diagonal of phys (taukdiag of cpacoord.f:300-309), from
+//   phys_ii = tau_ii + a_i ((T tau)_ii + (tau T)_ii) + a_i^2 (T tau T)_ii
+//             - a_i - a_i^2 T_ii.
+// One wave per (site, k); thread i < lk owns diagonal element i; the per-l
+// sums are taken in index order.
+// ---------------------------------------------------------------------------
+template <int MAXLK>
+__global__ __launch_bounds__(64) void phys_trace_kernel(
+    int lk, int nl, int sites, const double2* __restrict__ tauk, long long kstride,
+    const double2* __restrict__ tint, const double2* __restrict__ a,
+    const int* __restrict__ lidx, double2* __restrict__ out) {
+  __shared__ double2 st[MAXLK * MAXLK];  // tau, column-major
+  __shared__ double2 sT[MAXLK * MAXLK];  // T, column-major
+  __shared__ double2 sd[MAXLK];
+  const int site = blockIdx.x, b = blockIdx.y;
+  const long long nn = (long long)lk * lk;
+  const double2* __restrict__ tau = tauk + (long long)b * kstride + site * nn;
+  const double2* __restrict__ T = tint + site * nn;
+  for (int e = threadIdx.x; e < lk * lk; e += 64) {
+    st[e] = tau[e];
+    sT[e] = T[e];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < lk; i += 64) {
+    double2 tt = make_double2(0.0, 0.0), ttr = tt, ttt = tt;
+    for (int c = 0; c < lk; ++c) {
+      tt = cadd(tt, cmul(sT[i + c * lk], st[c + i * lk]));
+      ttr = cadd(ttr, cmul(st[i + c * lk], sT[c + i * lk]));
+    }
+    for (int c2 = 0; c2 < lk; ++c2) {
+      double2 row = make_double2(0.0, 0.0);  // (T tau)_{i,c2}
+      for (int c = 0; c < lk; ++c) row = cadd(row, cmul(sT[i + c * lk], st[c + c2 * lk]));
+      ttt = cadd(ttt, cmul(row, sT[c2 + i * lk]));
+    }
+    const double2 ai = a[(long long)site * lk + i], a2 = cmul(ai, ai);
+    double2 d = cadd(st[i + i * lk], cmul(ai, cadd(tt, ttr)));
+    d = cadd(d, cmul(a2, ttt));
+    d = csub(d, cadd(ai, cmul(a2, sT[i + i * lk])));
+    sd[i] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2* __restrict__ o = out + ((long long)b * sites + site) * nl;
+    for (int l = 0; l < nl; ++l) o[l] = make_double2(0.0, 0.0);
+    for (int i = 0; i < lk; ++i) o[lidx[i]] = cadd(o[lidx[i]], sd[i]);
+  }
+}
+
+// Full phys of one site block of the BZ sum (one work-group per site):
+//   X = tau + A (T tau),  phys = X + (X T) A - wsum (A + A T A),
+// wsum = sum of the BZ weights of the call (the constant term is affine in
+// the weights, so the result adds up over calls like the BZ sum).  X goes
+// through the scratch block x (site-major like tau).
+template <int MAXLK>
+__global__ __launch_bounds__(256) void phys_full_kernel(
+    int lk, const double2* __restrict__ taubz, const double2* __restrict__ tint,
+    const double2* __restrict__ a, double wsum, double2* __restrict__ x,
+    double2* __restrict__ out) {
+  __shared__ double2 sT[MAXLK * MAXLK];
+  const int site = blockIdx.x;
+  const long long nn = (long long)lk * lk;
+  const double2* __restrict__ tau = taubz + site * nn;
+  const double2* __restrict__ av = a + (long long)site * lk;
+  double2* __restrict__ xs = x + site * nn;
+  double2* __restrict__ os = out + site * nn;
+  for (int e = threadIdx.x; e < lk * lk; e += 256) sT[e] = tint[site * nn + e];
+  __syncthreads();
+  for (int e = threadIdx.x; e < lk * lk; e += 256) {
+    const int i = e % lk, j = e / lk;
+    double2 y = make_double2(0.0, 0.0);
+    for (int c = 0; c < lk; ++c) y = cadd(y, cmul(sT[i + c * lk], tau[c + j * lk]));
+    xs[e] = cadd(tau[e], cmul(av[i], y));
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < lk * lk; e += 256) {
+    const int i = e % lk, j = e / lk;
+    double2 y = make_double2(0.0, 0.0);
+    for (int c = 0; c < lk; ++c) y = cadd(y, cmul(xs[i + c * lk], sT[c + j * lk]));
+    double2 cm = cmul(cmul(av[i], sT[i + j * lk]), av[j]);
+    if (i == j) cm = cadd(cm, av[i]);
+    const double2 v = csub(cadd(xs[e], cmul(y, av[j])), make_double2(wsum * cm.x, wsum * cm.y));
+    os[e] = v;
+  }
+}

@@ -106,6 +106,15 @@ struct skkr_ctx {
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
   int graphs = 0;
   bool capturing = false;
+  // screening of the interface sites (physical tau, phystau.f)
+  bool have_screening = false;
+  int nl = 0;
+  z* dA = nullptr;       // sites x lk screening parameters per basis index
+  int* dLidx = nullptr;  // lk: l of each basis index
+  z* physx = nullptr;    // sites x lk x lk scratch
+  z* taubz_phys = nullptr;
+  z* phk = nullptr;      // per-k per-site per-l traces of a batch
+  int phk_cap = 0;
   std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by batch size nb
 };
 
@@ -722,13 +731,14 @@ static int auto_batch(skkr_ctx* c, int nk) {
   return std::max(1, std::min(B, nk));
 }
 
-int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
-                const double* w, double* tau_bz, double* tr_tau_k, int* iters_k,
-                int* status_k, skkr_stats* st) {
-  (void)e_re;
-  (void)e_im;
+// The k-loop of one energy; with phys, also the physical representation:
+// tau_bz_phys += phystau of this call's BZ sum, taukdiag = per-k per-l traces.
+static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, double* tau_bz,
+                      double* tr_tau_k, int* iters_k, int* status_k, skkr_stats* st,
+                      bool phys, double* tau_bz_phys, double* taukdiag) {
   if (!c || nk < 0 || (nk > 0 && (!k || !w)) || !tau_bz) return SKKR_ERR_ARG;
   if (!c->have_structure || !c->have_tinv) return SKKR_ERR_ARG;
+  if (phys && (!c->have_screening || !tau_bz_phys)) return SKKR_ERR_ARG;
   HIPCHK(hipSetDevice(c->cfg.device));
   auto t0 = std::chrono::steady_clock::now();
   CHK(ensure_batch(c, auto_batch(c, std::max(nk, 1))));
@@ -755,6 +765,7 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
   int rc = SKKR_OK;
   std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B);
   std::vector<double> h_tr(2 * sites * B);
+  std::vector<double> h_ph;
   for (int k0 = 0; k0 < nk; k0 += B) {
     const int nb = std::min(B, nk - k0);
     ++nbatches;
@@ -777,6 +788,25 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
       bz_accumulate_kernel<<<(ne + 255) / 256, 256, 0, c->stream>>>(ne, nb, c->tauk, (long long)tsz, c->dw, c->taubz);
       HIPCHK(hipGetLastError());
       c->launches++;
+    }
+    if (phys && taukdiag) {
+      const int nl = c->nl;
+      if (c->phk_cap < B) {
+        CHK(dalloc_t(c, &c->phk, (size_t)B * sites * nl));
+        c->phk_cap = B;
+      }
+      dim3 grid((unsigned)sites, (unsigned)nb);
+      if (lk <= 32)
+        phys_trace_kernel<32><<<grid, 64, 0, c->stream>>>(lk, nl, (int)sites, c->tauk, (long long)tsz,
+                                                          c->dTint, c->dA, c->dLidx, c->phk);
+      else
+        phys_trace_kernel<64><<<grid, 64, 0, c->stream>>>(lk, nl, (int)sites, c->tauk, (long long)tsz,
+                                                          c->dTint, c->dA, c->dLidx, c->phk);
+      HIPCHK(hipGetLastError());
+      c->launches++;
+      h_ph.resize((size_t)2 * nb * sites * nl);
+      HIPCHK(hipMemcpyAsync(h_ph.data(), c->phk, sizeof(z) * nb * sites * nl, hipMemcpyDeviceToHost,
+                            c->stream));
     }
     hipEventRecord(ev[4], c->stream);
     HIPCHK(hipMemcpyAsync(h_it.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -805,11 +835,41 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
       if (status_k) status_k[k0 + b] = code;
       if (code != SKKR_OK && rc == SKKR_OK) rc = code;
     }
+    if (phys && taukdiag) {
+      // taukdiag(ik, l, site), ik fastest (cpacoord.f taukdiag(mkpar,0:lmaxp,mintfc))
+      const int nl = c->nl;
+      for (int b = 0; b < nb; ++b)
+        for (size_t site = 0; site < sites; ++site)
+          for (int l = 0; l < nl; ++l) {
+            const size_t src = 2 * (((size_t)b * sites + site) * nl + l);
+            const size_t dst = 2 * ((size_t)(k0 + b) + (size_t)nk * ((size_t)l + (size_t)nl * site));
+            taukdiag[dst] = h_ph[src];
+            taukdiag[dst + 1] = h_ph[src + 1];
+          }
+    }
   }
   std::vector<double> hbz(2 * tsz);
   HIPCHK(hipMemcpyAsync(hbz.data(), c->taubz, sizeof(z) * tsz, hipMemcpyDeviceToHost, c->stream));
+  if (phys) {
+    double wsum = 0.0;
+    for (int i = 0; i < nk; ++i) wsum += w[i];
+    if (lk <= 32)
+      phys_full_kernel<32><<<(unsigned)sites, 256, 0, c->stream>>>(lk, c->taubz, c->dTint, c->dA, wsum,
+                                                                  c->physx, c->taubz_phys);
+    else
+      phys_full_kernel<64><<<(unsigned)sites, 256, 0, c->stream>>>(lk, c->taubz, c->dTint, c->dA, wsum,
+                                                                  c->physx, c->taubz_phys);
+    HIPCHK(hipGetLastError());
+    c->launches++;
+  }
+  std::vector<double> hph(phys ? 2 * tsz : 0);
+  if (phys)
+    HIPCHK(hipMemcpyAsync(hph.data(), c->taubz_phys, sizeof(z) * tsz, hipMemcpyDeviceToHost,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (size_t e = 0; e < 2 * tsz; ++e) tau_bz[e] += hbz[e];
+  if (phys)
+    for (size_t e = 0; e < 2 * tsz; ++e) tau_bz_phys[e] += hph[e];
   for (auto& e : ev) hipEventDestroy(e);
   if (st) {
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -831,6 +891,46 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
   }
   c->timing_detail = 0;
   return rc;
+}
+
+int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
+                const double* w, double* tau_bz, double* tr_tau_k, int* iters_k,
+                int* status_k, skkr_stats* st) {
+  (void)e_re;
+  (void)e_im;
+  return tau_bz_run(c, nk, k, w, tau_bz, tr_tau_k, iters_k, status_k, st, false, nullptr,
+                    nullptr);
+}
+
+int skkr_tau_bz_phys(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
+                     const double* w, double* tau_bz, double* tau_bz_phys, double* taukdiag,
+                     int* iters_k, int* status_k, skkr_stats* st) {
+  (void)e_re;
+  (void)e_im;
+  return tau_bz_run(c, nk, k, w, tau_bz, nullptr, iters_k, status_k, st, true, tau_bz_phys,
+                    taukdiag);
+}
+
+int skkr_set_screening(skkr_ctx* c, int nl, const int* l_of_index, const double* a) {
+  if (!c || nl < 1 || !l_of_index || !a) return SKKR_ERR_ARG;
+  const int lk = c->cfg.lk;
+  if (lk > 64) return SKKR_ERR_UNSUPPORTED;
+  for (int i = 0; i < lk; ++i)
+    if (l_of_index[i] < 0 || l_of_index[i] >= nl) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t sites = (size_t)c->cfg.n_pl * c->ns, tsz = sites * lk * lk;
+  if (!c->dA) {
+    CHK(dalloc_t(c, &c->dA, sites * lk));
+    CHK(dalloc_t(c, &c->dLidx, (size_t)lk));
+    CHK(dalloc_t(c, &c->physx, tsz));
+    CHK(dalloc_t(c, &c->taubz_phys, tsz));
+  }
+  if (nl > c->nl) c->phk_cap = 0;  // traces buffer sized by nl: reallocate on next use
+  HIPCHK(hipMemcpy(c->dA, a, sizeof(z) * sites * lk, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dLidx, l_of_index, sizeof(int) * lk, hipMemcpyHostToDevice));
+  c->nl = nl;
+  c->have_screening = true;
+  return SKKR_OK;
 }
 
 int skkr_set_option(skkr_ctx* c, const char* key, int value) {

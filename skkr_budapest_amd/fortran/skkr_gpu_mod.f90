@@ -68,5 +68,27 @@ module skkr_gpu
        complex(c_double_complex), intent(inout) :: tau_bz(*)
        type(c_ptr), value :: tr_tau_k, iters_k, status_k, stats
      end function skkr_tau_bz
+
+     ! screening alpha_l(site) expanded per basis index (lml / ldex), phystau.f
+     integer(c_int) function skkr_set_screening(ctx, nl, l_of_index, a) &
+          bind(C, name="skkr_set_screening")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nl
+       integer(c_int), intent(in) :: l_of_index(*)
+       complex(c_double_complex), intent(in) :: a(*)
+     end function skkr_set_screening
+
+     ! k-loop plus physical tau: tau_bz_phys += phystau(BZ sum), taukdiag(ik,l,li)
+     integer(c_int) function skkr_tau_bz_phys(ctx, e_re, e_im, nk, k, w, tau_bz, &
+          tau_bz_phys, taukdiag, iters_k, status_k, stats) bind(C, name="skkr_tau_bz_phys")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       real(c_double), value :: e_re, e_im
+       integer(c_int), value :: nk
+       real(c_double), intent(in) :: k(*), w(*)
+       complex(c_double_complex), intent(inout) :: tau_bz(*), tau_bz_phys(*)
+       type(c_ptr), value :: taukdiag, iters_k, status_k, stats
+     end function skkr_tau_bz_phys
   end interface
 end module skkr_gpu

@@ -32,7 +32,8 @@ FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
 SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structure",
            "skkr_set_tinv", "skkr_tau_bz", "skkr_pl_blocks", "skkr_gjinv_batch",
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
-           "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune")
+           "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
+           "skkr_tau_bz_phys")
 
 
 class SkkrError(RuntimeError):
@@ -101,6 +102,8 @@ def lib():
         L.skkr_tau2d_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
         L.skkr_set_option.argtypes = [V, ctypes.c_char_p, I]
         L.skkr_autotune.argtypes = [V, ctypes.POINTER(Tuning)]
+        L.skkr_set_screening.argtypes = [V, I, V, V]
+        L.skkr_tau_bz_phys.argtypes = [V, D, D, I, V, V, V, V, V, V, V, ctypes.POINTER(Stats)]
         _LIB = L
     return _LIB
 
@@ -119,6 +122,7 @@ class Context:
     def __init__(self, lk, sites_per_pl, n_pl, errmax=1e-8, itermax=50,
                  precision=FP64, fp64_dec_below_imE=0.02, device=0, batch=0):
         self.lk, self.s, self.P = lk, sites_per_pl, n_pl
+        self.nl = 0  # trace groups of the screening (set_screening)
         self.n = lk * sites_per_pl
         cfg = Config(lk, sites_per_pl, n_pl, errmax, itermax, itermax, precision,
                      fp64_dec_below_imE, device, batch)
@@ -198,6 +202,38 @@ class Context:
         acc[...] = from_colmajor(buf)
         return {"tau_bz": acc, "trk": trk, "iters": iters, "status": status,
                 "stats": st.as_dict(), "rc": rc}
+
+    def set_screening(self, a, l_of_index, nl=None):
+        """a: (sites, lk) complex screening per basis index; l_of_index: (lk,)."""
+        a = np.ascontiguousarray(a, np.complex128)
+        li = np.ascontiguousarray(l_of_index, np.int32)
+        nl = int(nl if nl is not None else li.max() + 1)
+        self._check(lib().skkr_set_screening(self._h, nl, _p(li), _p(a)), "skkr_set_screening")
+        self.nl = nl
+
+    def tau_bz_phys(self, E, k, w, want_taukdiag=True, raise_on_error=True):
+        """skkr_tau_bz with the physical representation: returns tau_bz,
+        tau_bz_phys (sites, lk, lk) and taukdiag (nk, sites, nl)."""
+        k = np.ascontiguousarray(k, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        nk = len(w)
+        ns = self.P * self.s
+        acc = np.zeros((ns, self.lk, self.lk), np.complex128)
+        buf, pbuf = colmajor(acc), colmajor(acc)
+        tkd = np.zeros((ns, self.nl, nk), np.complex128) if want_taukdiag else None
+        iters = np.zeros((nk, 2), np.int32)
+        status = np.zeros(nk, np.int32)
+        st = Stats()
+        rc = lib().skkr_tau_bz_phys(self._h, float(np.real(E)), float(np.imag(E)), nk, _p(k),
+                                    _p(w), _p(buf), _p(pbuf), _p(tkd), _p(iters), _p(status),
+                                    ctypes.byref(st))
+        if raise_on_error:
+            self._check(rc, "skkr_tau_bz_phys")
+        out = {"tau_bz": from_colmajor(buf), "tau_bz_phys": from_colmajor(pbuf), "iters": iters,
+               "status": status, "stats": st.as_dict(), "rc": rc}
+        if tkd is not None:
+            out["taukdiag"] = np.ascontiguousarray(np.transpose(tkd, (2, 0, 1)))
+        return out
 
     # ---- per-routine entry points ----------------------------------------
     def pl_blocks(self, k):

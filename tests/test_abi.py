@@ -74,3 +74,14 @@ def test_mlp_argument_validation_without_device():
     assert lib.skkr_mlp_create(ctypes.byref(h), 0, 64, 0, 1) == 1     # nin < 1
     assert lib.skkr_mlp_create(ctypes.byref(h), 8, 32, 0, 1) == 1     # nhid != 64
     assert lib.skkr_mlp_create(None, 8, 64, 0, 1) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_screening_entry_points_validate_arguments_without_device():
+    lib = ctypes.CDLL(LIB)
+    lib.skkr_set_screening.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+    assert lib.skkr_set_screening(None, 3, None, None) == 1      # SKKR_ERR_ARG
+    lib.skkr_tau_bz_phys.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int] + [ctypes.c_void_p] * 8
+    assert lib.skkr_tau_bz_phys(None, 0.5, 0.1, 0, *([None] * 8)) == 1

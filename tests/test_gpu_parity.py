@@ -337,6 +337,64 @@ def test_smaller_call_after_a_larger_batch(gpu):
     np.testing.assert_array_equal(small["tau_bz"], fresh["tau_bz"])
 
 
+def _screening(sy, nl, seed):
+    rng = np.random.default_rng(seed)
+    lk, ns = sy.cfg.lk, sy.cfg.P * sy.cfg.s
+    lidx = np.sort(rng.integers(0, nl, size=lk))
+    lidx[:nl] = np.arange(nl)
+    lidx = np.sort(lidx)
+    alpha_l = 0.2 * (rng.normal(size=(ns, nl)) + 1j * rng.normal(size=(ns, nl)))
+    return alpha_l[:, lidx], lidx      # per-index parameters, like lml / ldex
+
+
+@pytest.mark.parametrize("lk,s,P,tmodel", [(8, 2, 3, "herm"), (32, 1, 2, "herm"),
+                                           (36, 1, 2, "bdg")])
+def test_physical_tau_matches_oracle(gpu, lk, s, P, tmodel):
+    """skkr_tau_bz_phys: phystau of the BZ sum and the per-k per-l traces of
+    phystau(tau(k)) (cpacoord.f:294-311) against the oracle's phystau, whose
+    output is pinned to the reference's phystau in test_oracle.py."""
+    sy = small_sys(lk=lk, s=s, P=P, r2max=4, tmodel=tmodel, E=0.35 + 0.04j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(4)
+    nl = 3
+    a, lidx = _screening(sy, nl, seed=lk)
+    ref = oracle.tau_bz(sy, E, k, w, nworkers=8, want_tau_k=True)
+    tint = sy.tinv(E)[0]
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ctx.set_screening(a, lidx, nl)
+        r = ctx.tau_bz_phys(E, k, w)
+        plain = ctx.tau_bz(E, k, w)
+    np.testing.assert_array_equal(r["tau_bz"], plain["tau_bz"])
+    ns = P * s
+    phys_ref = np.stack([oracle.phystau(ref["tau_bz"][i], tint[i], tint[i], a[i]) for i in range(ns)])
+    assert rel(r["tau_bz_phys"], phys_ref) < TOL
+    tkd_ref = np.array([[oracle.phys_traces(ref["tau_k"][ik, i], tint[i], a[i], lidx, nl)
+                         for i in range(ns)] for ik in range(len(w))])
+    assert r["taukdiag"].shape == (len(w), ns, nl)
+    assert rel(r["taukdiag"], tkd_ref) < TOL
+    # the BZ-weighted per-k traces are the traces of the physical BZ sum
+    tr_bz = np.array([[np.trace(phys_ref[i][np.ix_(lidx == l, lidx == l)]) for l in range(nl)]
+                      for i in range(ns)])
+    assert rel(np.einsum("k,kil->il", w, r["taukdiag"]), tr_bz) < TOL
+
+
+def test_physical_tau_adds_up_over_calls(gpu):
+    sy = small_sys(lk=8, s=2, P=3, r2max=4, E=0.35 + 0.04j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(4)
+    a, lidx = _screening(sy, 3, seed=1)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ctx.set_screening(a, lidx, 3)
+        whole = ctx.tau_bz_phys(E, k, w)
+        p1 = ctx.tau_bz_phys(E, k[:5], w[:5])
+        p2 = ctx.tau_bz_phys(E, k[5:], w[5:])
+    assert rel(p1["tau_bz_phys"] + p2["tau_bz_phys"], whole["tau_bz_phys"]) < 1e-13
+    np.testing.assert_array_equal(np.concatenate([p1["taukdiag"], p2["taukdiag"]]),
+                                  whole["taukdiag"])
+
+
 def test_set_option_rejects_unknown(gpu):
     with gpu.Context(lk=4, sites_per_pl=1, n_pl=1) as ctx:
         with pytest.raises(gpu.SkkrError):

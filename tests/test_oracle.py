@@ -207,3 +207,53 @@ def test_round_robin_workers_sum_like_allreduce():
     np.testing.assert_array_equal(a["iters"], b["iters"])
     # constant-tau invariance: BZ weights sum to one
     assert abs(w.sum() - 1.0) < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# phystau: screened -> physical tau (the reference's own phystau.f outputs)
+# ---------------------------------------------------------------------------
+def _index_maps(lmax):
+    """Basis index -> l: lm order (l, m = -l..l); kappa-mu order with
+    2(2l+1) states per l; electron half then hole half for the BdG basis."""
+    lm = np.concatenate([np.full(2 * l + 1, l) for l in range(lmax + 1)])
+    kmy = np.concatenate([np.full(2 * (2 * l + 1), l) for l in range(lmax + 1)])
+    return {"nrsp": lm, "rsp": kmy, "dbogo": np.concatenate([kmy, kmy])}
+
+
+@pytest.mark.parametrize("variant", ["nrsp", "rsp", "dbogo"])
+@pytest.mark.parametrize("idiag", [1, 0])
+def test_phystau_matches_reference_golden(variant, idiag):
+    d = np.load(os.path.join(GOLD, "phystau.npz"))
+    lmap = _index_maps(int(d["lmax"]))[variant]
+    key = f"{variant}_idiag{idiag}"
+    for c in range(len(d[key + "_out"])):
+        al, ah = d[key + "_alpha"][c], d[key + "_alphah"][c]
+        a = al[lmap]
+        if variant == "dbogo":
+            half = len(lmap) // 2
+            a = np.concatenate([al[lmap[:half]], ah[lmap[half:]]])
+        got = oracle.phystau(d[key + "_tau"][c], d[key + "_tinv1"][c], d[key + "_tinv2"][c], a,
+                             idiag)
+        ref = d[key + "_out"][c]
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_phystau_closed_forms():
+    rng = np.random.default_rng(7)
+    n = 6
+    tau = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    t = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    # no screening: the physical tau is the screened one
+    assert np.array_equal(oracle.phystau(tau, t, t, np.zeros(n)), tau)
+    # 1x1: (1 + a t) tau (1 + t a) - (a + a t a)
+    a, tt, x = 0.3 - 0.2j, 1.1 + 0.4j, -0.7 + 2.0j
+    got = oracle.phystau(np.array([[x]]), np.array([[tt]]), np.array([[tt]]), np.array([a]))[0, 0]
+    assert abs(got - ((1 + a * tt) * x * (1 + tt * a) - (a + a * tt * a))) < 1e-14
+    # affine in tau: for weights summing to one, phystau of the weighted sum
+    # is the weighted sum of phystau (the BZ sum commutes with it)
+    taus = rng.normal(size=(3, n, n)) + 1j * rng.normal(size=(3, n, n))
+    w = np.array([0.2, 0.5, 0.3])
+    av = rng.normal(size=n) + 1j * rng.normal(size=n)
+    lhs = oracle.phystau((w[:, None, None] * taus).sum(0), t, t, av)
+    rhs = sum(wi * oracle.phystau(ti, t, t, av) for wi, ti in zip(w, taus))
+    assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(lhs).max()
