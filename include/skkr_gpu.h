@@ -142,7 +142,11 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it
  *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 128
  *   "inv_kernel" 0 blocked MFMA Gauss-Jordan | 1 row-wise Gauss-Jordan
- *                (different rounding; not bitwise equal to 0)
+ *                (different rounding; not bitwise equal to 0) | 2 blocked
+ *                without row exchanges inside the 16x16 diagonal blocks: for
+ *                accretive matrices only (positive-definite anti-Hermitian
+ *                part), larger rounding error (also SKKR_INV_KERNEL=rowwise
+ *                or nopivot in the environment)
  *   "gemm_3m"   0 four-product | 1 three-product (Gauss) complex GEMM on the
  *               128 tile: 3/4 of the MFMA work, normwise FP64 accuracy,
  *               different rounding (also SKKR_GEMM_3M=1 in the environment) */

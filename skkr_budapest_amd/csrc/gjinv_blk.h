@@ -72,7 +72,7 @@ __device__ __forceinline__ void gj_update(double2& dq, int r, int p, bool pivcol
 // would be lowered to scratch memory).  Pivot search by lane reads, pivot
 // row and column by shuffles.  Writes the unscrambled inverse to
 // dinv[col * DP + row]; sets *sing on a zero or NaN pivot.
-template <int DP>
+template <int DP, bool PIV>
 __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2& d2, double2& d3,
                                                int lane, double2* __restrict__ dinv, int* sperm,
                                                int* spinv, int* sing) {
@@ -85,9 +85,11 @@ __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2
     const int csrc = l4 * 16 + it;
     const double2 f0 = shfl2(d0, csrc), f1 = shfl2(d1, csrc);
     const double2 f2 = shfl2(d2, csrc), f3 = shfl2(d3, csrc);
+    int p = it;
+    if constexpr (PIV) {
     // candidates of column it: rows l4 + 4q of lane l4*16 + it
     double best = -1.0;
-    int p = 16;
+    p = 16;
     {
       const double m0 = d0.x * d0.x + d0.y * d0.y, m1 = d1.x * d1.x + d1.y * d1.y;
       const double m2 = d2.x * d2.x + d2.y * d2.y, m3 = d3.x * d3.x + d3.y * d3.y;
@@ -117,11 +119,15 @@ __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2
       sperm[it] = p;
       spinv[p] = it;
     }
+    }
     // pivot row p of this lane's column: lane (p&3)*16 + l16, element p>>2
     const int qp = p >> 2;
     const double2 mine = qp == 0 ? d0 : qp == 1 ? d1 : qp == 2 ? d2 : d3;
     const double2 prow = shfl2(mine, (p & 3) * 16 + l16);
     const double2 piv = make_double2(readlane_f64(prow.x, it), readlane_f64(prow.y, it));
+    if constexpr (!PIV) {
+      if (!(piv.x * piv.x + piv.y * piv.y > 0.0)) bad = true;  // zero or NaN pivot
+    }
     const double2 qv = pivot_recip(piv);
     const bool pivcol = (l16 == it);
     const double2 rn = pivcol ? qv : cmul(prow, qv);
@@ -131,17 +137,24 @@ __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2
     gj_update(d3, l4 + 12, p, pivcol, f3, rn);
   }
   if (bad && lane == 0) *sing = 1;
-  // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  const int cc = sperm[l16];
-  dinv[cc * DP + spinv[l4]] = d0;
-  dinv[cc * DP + spinv[l4 + 4]] = d1;
-  dinv[cc * DP + spinv[l4 + 8]] = d2;
-  dinv[cc * DP + spinv[l4 + 12]] = d3;
+  if constexpr (PIV) {
+    // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int cc = sperm[l16];
+    dinv[cc * DP + spinv[l4]] = d0;
+    dinv[cc * DP + spinv[l4 + 4]] = d1;
+    dinv[cc * DP + spinv[l4 + 8]] = d2;
+    dinv[cc * DP + spinv[l4 + 12]] = d3;
+  } else {
+    dinv[l16 * DP + l4] = d0;
+    dinv[l16 * DP + l4 + 4] = d1;
+    dinv[l16 * DP + l4 + 8] = d2;
+    dinv[l16 * DP + l4 + 12] = d3;
+  }
 }
 
-template <int T, int TI, int TJ>
+template <int T, int TI, int TJ, bool PIV = true>
 __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel(InvArgs args) {
   constexpr int N = 16 * T;
   constexpr int WC = T / TJ;
@@ -215,7 +228,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             for (int q = 0; q < 4; ++q) d[q] = make_double2(mre[i][j][q], mim[i][j][q]);
           }
         }
-    if (own) diag_inverse16<DP>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
+    if (own) diag_inverse16<DP, PIV>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
   }
 
 #pragma unroll 1
@@ -325,7 +338,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
             }
           }
         }
-      if (own) diag_inverse16<DP>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
+      if (own) diag_inverse16<DP, PIV>(d[0], d[1], d[2], d[3], lane, Dinv, sperm, spinv, &s_sing);
     }
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {

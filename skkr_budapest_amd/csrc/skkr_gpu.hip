@@ -346,6 +346,19 @@ static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
     c->launches++;
     return SKKR_OK;
   }
+  if (c->inv_kernel == 2) {  // blocked, no pivoting inside the diagonal blocks
+    if (a.n <= 32)
+      gjinv_blk_f64_kernel<2, 2, 2, false><<<nb, 64, 0, c->stream>>>(a);
+    else if (a.n <= 64)
+      gjinv_blk_f64_kernel<4, 2, 2, false><<<nb, 256, 0, c->stream>>>(a);
+    else if (a.n <= 128)
+      gjinv_blk_f64_kernel<8, 2, 4, false><<<nb, 512, 0, c->stream>>>(a);
+    else
+      return SKKR_ERR_UNSUPPORTED;
+    HIPCHK(hipGetLastError());
+    c->launches++;
+    return SKKR_OK;
+  }
   if (a.n <= 32) {
     gjinv_f64_kernel<8, 8, 4, 4><<<nb, 64, 0, c->stream>>>(a);
   } else if (a.n <= 64) {
@@ -636,7 +649,8 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   HIPCHK(hipSetDevice(cfg->device));
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
-  if (const char* e = getenv("SKKR_INV_KERNEL")) c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : 0;
+  if (const char* e = getenv("SKKR_INV_KERNEL"))
+    c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
   if (const char* e = getenv("SKKR_GEMM_3M")) c->gemm_3m = (atoi(e) != 0) ? 1 : 0;
   if (const char* e = getenv("SKKR_GEMM_TILE")) {
     const int t = atoi(e);
@@ -947,7 +961,7 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     return SKKR_OK;
   }
   if (strcmp(key, "inv_kernel") == 0) {
-    if (value != 0 && value != 1) return SKKR_ERR_ARG;
+    if (value < 0 || value > 2) return SKKR_ERR_ARG;
     c->inv_kernel = value;
     drop_graphs(c);
     return SKKR_OK;

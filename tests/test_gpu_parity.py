@@ -81,6 +81,43 @@ def test_gjinv_matches_numpy(gpu, n, kernel, monkeypatch):
         assert (st == 4).all()
 
 
+@pytest.mark.parametrize("n", [9, 36, 64, 128])
+def test_gjinv_without_diagonal_pivoting_on_accretive_matrices(gpu, n):
+    """inv_kernel 2 (no row exchanges inside the 16x16 diagonal blocks) on
+    matrices of the layer problem's class: E - H with Im E > 0, whose
+    anti-Hermitian part is positive definite, so every leading block is
+    non-singular.  The tolerance is looser than with pivoting."""
+    rng = np.random.default_rng(500 + n)
+    a = rng.normal(size=(6, n, n)) + 1j * rng.normal(size=(6, n, n))
+    h = (a + np.conj(np.swapaxes(a, 1, 2))) / 2
+    m = (0.3 + 0.02j) * np.eye(n) - h
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        ctx.set_option("inv_kernel", 2)
+        inv, st = ctx.gjinv(m)
+        assert (st == 0).all()
+        ref = np.linalg.inv(m)
+        assert rel(inv, ref) < 1e-10
+        z, st = ctx.gjinv(np.zeros((2, n, n)))
+        assert (st == 4).all()
+
+
+def test_config1_sample_without_diagonal_pivoting(gpu):
+    """Config 1 with inv_kernel 2 in the decimation and sweeps."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.array([0, 1300, 2048, 3500])
+    ks, ws = k[idx], w[idx]
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=8)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_option("inv_kernel", 2)
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, ks, ws)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL_CFG1
+
+
 def test_pl_blocks_match_oracle(gpu):
     sy = S.make_system(S.small_config(lk=6, s=3, P=2, r2max=8))
     rng = np.random.default_rng(3)
