@@ -179,3 +179,142 @@ def tau_bz(system, E, k, w, nworkers=1, want_tau_k=False, itermax=None,
     if tau_k is not None:
         out["tau_k"] = _uncm(tau_k)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Layer structure constants (kkrstr_oracle.c): Kambe Ewald sum, direct
+# reciprocal sum and screening of skkr_kpar_v6/main/gstore.f.
+
+def _kkr_lib():
+    L = lib()
+    if not hasattr(L, "_kkr_ready"):
+        P = ctypes.c_void_p
+        i, d = ctypes.c_int, ctypes.c_double
+        L.orc_cerf_v.argtypes = [i, P, P]
+        L.orc_incgamma_p.argtypes = [i, P, P]
+        L.orc_delkam_p.argtypes = [i, P, P, P]
+        L.orc_ilkam_p.argtypes = [i, P, d, P, P]
+        L.orc_lattice_new.argtypes = [P, P, i]
+        L.orc_lattice_new.restype = P
+        L.orc_lattice_info.argtypes = [P, P, P]
+        L.orc_lattice_vectors.argtypes = [P, P, P]
+        L.orc_lattice_free.argtypes = [P]
+        L.orc_kamstr_p.argtypes = [P, i, P, d, d, P, d, P, P]
+        L.orc_directg_p.argtypes = [P, i, d, d, P, P, P, P, P]
+        L.orc_gstore_p.argtypes = [P, i, i, i, P, P, d, d, P, d, P]
+        L.orc_gaunt.argtypes = [i, P]
+        L.orc_ylmc.argtypes = [d, d, d, i, P]
+        for f in (L.orc_kamstr_p, L.orc_directg_p, L.orc_gstore_p):
+            f.restype = ctypes.c_int
+        L._kkr_ready = True
+    return L
+
+
+def cerf(z):
+    """Faddeeva w(z) as cerf.f computes it (its series, thresholds and
+    single-precision constants)."""
+    z = np.ascontiguousarray(np.atleast_1d(z), np.complex128)
+    w = np.empty_like(z)
+    _kkr_lib().orc_cerf_v(len(z), _p(z), _p(w))
+    return w
+
+
+def incgamma(n, z):
+    g = np.empty(n + 1, np.complex128)
+    zz = np.array([z], np.complex128)
+    _kkr_lib().orc_incgamma_p(n, _p(zz), _p(g))
+    return g
+
+
+def delkam(n, x, zz):
+    out = np.empty(max(n + 1, 2), np.complex128)
+    a, b = np.array([x], np.complex128), np.array([zz], np.complex128)
+    _kkr_lib().orc_delkam_p(n, _p(a), _p(b), _p(out))
+    return out[:n + 1]
+
+
+def ilkam(l, alpha, t, kappa):
+    out = np.empty(l + 1, np.complex128)
+    a, k = np.array([alpha], np.complex128), np.array([kappa], np.complex128)
+    _kkr_lib().orc_ilkam_p(l, _p(a), float(t), _p(k), _p(out))
+    return out
+
+
+def gaunt_table(lmax):
+    """gauntcalc.f: gau over (l1,m1,l2,m2,l,m), l <= 2 lmax."""
+    nl2, lg = (lmax + 1) ** 2, (2 * lmax + 1) ** 2
+    g = np.empty(nl2 * nl2 * lg)
+    _kkr_lib().orc_gaunt(lmax, _p(g))
+    return g.reshape(nl2, nl2, lg)
+
+
+def ylmc(v, lmax):
+    y = np.empty((lmax + 1) ** 2, np.complex128)
+    _kkr_lib().orc_ylmc(float(v[0]), float(v[1]), float(v[2]), lmax, _p(y))
+    return y
+
+
+class Lattice:
+    """2D direct and reciprocal lattice of vector2d.f (mdir vectors each),
+    the reciprocal vectors sorted into shells around the origin."""
+
+    def __init__(self, a1, a2, mdir=20000):
+        a = np.array(a1, np.float64)
+        b = np.array(a2, np.float64)
+        self._L = _kkr_lib().orc_lattice_new(_p(a), _p(b), mdir)
+        d = np.empty(8)
+        n = np.empty(3, np.int32)
+        _kkr_lib().orc_lattice_info(self._L, _p(d), _p(n))
+        self.distr, self.distk, self.vol, self.volbz = d[:4]
+        self.b1, self.b2 = d[4:6].copy(), d[6:8].copy()
+        self.maxr, self.maxk, self.maxshk = (int(x) for x in n)
+
+    def vectors(self):
+        r = np.empty((self.maxr, 2))
+        g = np.empty((self.maxk, 2))
+        _kkr_lib().orc_lattice_vectors(self._L, _p(r), _p(g))
+        return r, g
+
+    def __del__(self):
+        try:
+            _kkr_lib().orc_lattice_free(self._L)
+        except Exception:
+            pass
+
+    def kamstr(self, lmax, e, k, sep, eta):
+        """Kambe structure constant of one layer pair (sep = 3-vector);
+        returns (alm [lm, lm'], (G shells, R shells), status)."""
+        nl2 = (lmax + 1) ** 2
+        alm = np.zeros(nl2 * nl2, np.complex128)
+        ts = np.array([np.linalg.norm(sep), *sep], np.float64)
+        ee = np.array([e], np.complex128)
+        nsh = np.zeros(2, np.int32)
+        st = _kkr_lib().orc_kamstr_p(self._L, lmax, _p(ee), float(k[0]), float(k[1]), _p(ts),
+                                      float(eta), _p(alm), _p(nsh))
+        return alm.reshape(nl2, nl2).T, tuple(int(x) for x in nsh), st
+
+    def directg(self, lmax, e, k, sep):
+        nl2 = (lmax + 1) ** 2
+        gp = np.zeros(nl2 * nl2, np.complex128)
+        gm = np.zeros(nl2 * nl2, np.complex128)
+        sp = np.array(sep, np.float64)
+        ee = np.array([e], np.complex128)
+        nsh = np.zeros(1, np.int32)
+        st = _kkr_lib().orc_directg_p(self._L, lmax, float(k[0]), float(k[1]), _p(sp), _p(ee),
+                                       _p(gp), _p(gm), _p(nsh))
+        return gp.reshape(nl2, nl2).T, gm.reshape(nl2, nl2).T, int(nsh[0]), st
+
+    def gstore(self, lmax, s, nextra, cvec_pl, alpha, e, k, eta):
+        """g2d of gstore.f (bulkgeo) for one k: array [icpl, icount, jj+s, n, m]
+        (the reverse of g2d(m, n, jj, icount, icpl)); alpha[layer, l]."""
+        nl2 = (lmax + 1) ** 2
+        cv = np.ascontiguousarray(cvec_pl, np.float64)
+        al = np.ascontiguousarray(alpha, np.complex128)
+        assert cv.shape == (s + 1, 3) and al.shape == (s, lmax + 1)
+        out = np.zeros(nl2 * nl2 * (2 * s + 1) * s * 3, np.complex128)
+        ee = np.array([e], np.complex128)
+        st = _kkr_lib().orc_gstore_p(self._L, lmax, s, nextra, _p(cv), _p(al), float(k[0]),
+                                      float(k[1]), _p(ee), float(eta), _p(out))
+        if st:
+            raise RuntimeError(f"orc_gstore status {st}")
+        return out.reshape(3, s, 2 * s + 1, nl2, nl2)
