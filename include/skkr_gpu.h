@@ -37,7 +37,9 @@ enum {
   SKKR_ERR_SINGULAR = 4,     /* zero pivot in an inversion */
   SKKR_ERR_SGF_DIVERGED = 5, /* decimation |alpha|_F > 1e10 (surfgf.f:89-94) */
   SKKR_ERR_SGF_ITERMAX = 6,  /* decimation iter > itermax (surfgf.f:100-103) */
-  SKKR_ERR_UNSUPPORTED = 7   /* size/mode not built */
+  SKKR_ERR_UNSUPPORTED = 7,  /* size/mode not built */
+  SKKR_ERR_STR_NOCONV = 8    /* structure-constant shell sum not converged
+                                (kamstr.f:358-359,388-389, directg.f:128-131) */
 };
 
 /* precision modes */
@@ -202,6 +204,53 @@ int skkr_dugo_batch(skkr_ctx *ctx, char side, int nb, const double *m00,
 int skkr_tau2d_batch(skkr_ctx *ctx, int nb, const double *m00,
                      const double *m01, const double *m10, const double *x,
                      const double *y, double w, double *tau, int *status);
+
+/* ------------------------------------------------------------------------
+ * Layer structure constants (the G(k) producer of the reference):
+ * skkr_kpar_v6/main/gstore.f:6-7
+ *   gstore(park, ce, factor, bulkgeo, side, lmax, nintfc_true, g2d)
+ * for the bulk-like geometry (bulkgeo, gstore.f:99-160), called per k at
+ * nrsp-scf/cpacoord.f:205,217,250.  Kambe's Ewald sum (kamstr.f) for close
+ * layers, the direct reciprocal sum (directg.f) for layers with
+ * |dz| * min|b| >= 5, screening G' = G (1 - alpha G)^-1 over (2 nextra + 3)
+ * principal layers (scrstr.f), and the layer-tridiagonal blocks g2d.
+ * Non-relativistic lm basis, nl2 = (lmax+1)^2 per layer; lmax <= 3 and
+ * nl2 (2 nextra + 3) s <= 256.
+ * ------------------------------------------------------------------------ */
+typedef struct skkr_str skkr_str;
+
+typedef struct {
+  int lmax;          /* l cutoff of the structure constants (LMAX) */
+  int s;             /* layers per principal layer (ninprc) */
+  int nextra;        /* extra principal layers each side for the screening
+                        (NEXTRA, readgeom.f) */
+  double eta;        /* Ewald parameter (ETA, readini.f:161; 'factor' of
+                        kamstr.f) */
+  double a1[2], a2[2]; /* 2D Bravais basis in a.u. (common/brav2d/,
+                          primkr2d.f) */
+  int device;        /* HIP device */
+  int batch;         /* k-points per batch (0: 256) */
+} skkr_str_config;
+
+/* cvec: 3 x (s+1) layer positions in a.u. (x, y, z per layer): the layers of
+ * one principal layer and the first layer of the next one (gstore.f:105-122
+ * builds the stack from these).  The direct and reciprocal lattices are
+ * those of vector2d.f / genlatt2d.f with mdir = mrec = 20000 vectors. */
+int skkr_str_create(skkr_str **out, const skkr_str_config *cfg,
+                    const double *cvec);
+void skkr_str_destroy(skkr_str *sc);
+
+/* g2d for nk k-points at energy E:
+ * alpha  complex (lmax+1) x s, l fastest: the screening parameter of each
+ *        layer of the principal layer (alphalkkr(l, layer) for side 'L',
+ *        alpharkkr for 'R'; alphamat.f);
+ * k      2 x nk (park per k);
+ * g2d    complex, per k nl2 x nl2 x (2s+1) x s x 3 column-major: the
+ *        reference's g2d(m, n, jj = -s..s, icount = 1..s, icpl = 0..2);
+ * status per k (optional): SKKR_OK, SKKR_ERR_STR_NOCONV or
+ *        SKKR_ERR_SINGULAR. */
+int skkr_gstore_batch(skkr_str *sc, double e_re, double e_im, const double *alpha,
+                      int nk, const double *k, double *g2d, int *status);
 
 #ifdef __cplusplus
 }

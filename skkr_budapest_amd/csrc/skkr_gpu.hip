@@ -23,6 +23,8 @@
 #include "../../include/skkr_gpu.h"
 #include "kernels_f64.h"
 #include "gjinv_blk.h"
+#include "kkrlat.h"
+#include "kkrstr.h"
 
 #define HIPCHK(x)                                                          \
   do {                                                                     \
@@ -81,6 +83,7 @@ struct skkr_ctx {
   // D = A11^-1 A12, and views of sub-blocks
   z *Asc = nullptr, *Csc = nullptr, *Dsc = nullptr;
   PtrArr p_Asc, p_A21, p_A12, p_A22, p_Csc, p_Dsc;
+  int invB = 0;  // batch capacity of the 2x2 block inversion scratch
   z** dv_out[4] = {nullptr, nullptr, nullptr, nullptr};  // out + offsets (B11, B21, B12, B22)
   int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
       *nactive = nullptr, *status_sw = nullptr;
@@ -183,7 +186,7 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_t(c, &c->Xc, B * nn));
   CHK(dalloc_t(c, &c->tauk, (size_t)B * P * ns * lk * lk / (ns ? 1 : 1)));
   CHK(dalloc_t(c, &c->trk, (size_t)B * P * ns));
-  if (n > 128) {
+  if (n > 128 && c->invB < B) {
     const size_t h1 = 128, h2 = n - 128;
     CHK(dalloc_t(c, &c->Asc, B2 * nn));
     CHK(dalloc_t(c, &c->Csc, B2 * h1 * h2));
@@ -215,7 +218,7 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(fill(&c->p_AYBY, [&](size_t j) { return c->AYBY + j * 2 * nn; }, B2));
   // self-energy outputs: Y_1 into Ycur, X_P into Xc
   CHK(fill(&c->p_sigma, [&](size_t j) { return j < Bs ? c->Ycur + j * nn : c->Xc + (j - Bs) * nn; }, B2));
-  if (n > 128) {
+  if (n > 128 && c->invB < B) {
     const size_t h1 = 128, h2 = n - 128;
     CHK(fill(&c->p_Asc, [&](size_t j) { return c->Asc + j * nn; }, B2));
     CHK(fill(&c->p_A21, [&](size_t j) { return c->Asc + j * nn + h1; }, B2));
@@ -223,6 +226,7 @@ static int ensure_batch(skkr_ctx* c, int B) {
     CHK(fill(&c->p_A22, [&](size_t j) { return c->Asc + j * nn + h1 + h1 * n; }, B2));
     CHK(fill(&c->p_Csc, [&](size_t j) { return c->Csc + j * h1 * h2; }, B2));
     CHK(fill(&c->p_Dsc, [&](size_t j) { return c->Dsc + j * h1 * h2; }, B2));
+    c->invB = B;
   }
   CHK(fill(&c->p_M00, [&](size_t j) { return c->M00 + j * nn; }, Bs));
   CHK(fill(&c->p_M01, [&](size_t j) { return c->M01 + j * nn; }, Bs));
@@ -242,6 +246,31 @@ static int ensure_batch(skkr_ctx* c, int B) {
     CHK(fill(&c->p_tr[p], [&](size_t j) { return c->trk + j * P * ns + p * ns; }, Bs));
   }
   c->B = B;
+  return SKKR_OK;
+}
+
+// scratch of the 2x2 block inversion only (n > 128), for contexts that
+// invert matrices outside the k-loop
+static int ensure_inverse(skkr_ctx* c, int B) {
+  const int n = c->n;
+  if (n <= 128 || c->invB >= B) return SKKR_OK;
+  const size_t nn = c->nn, B2 = 2 * (size_t)B, h1 = 128, h2 = n - 128;
+  CHK(dalloc_t(c, &c->Asc, B2 * nn));
+  CHK(dalloc_t(c, &c->Csc, B2 * h1 * h2));
+  CHK(dalloc_t(c, &c->Dsc, B2 * h1 * h2));
+  for (int v = 0; v < 4; ++v) CHK(dalloc_t(c, &c->dv_out[v], B2));
+  auto fill = [&](PtrArr* pa, auto fn, size_t cnt) -> int {
+    std::vector<z*> t(cnt);
+    for (size_t j = 0; j < cnt; ++j) t[j] = fn(j);
+    return make_ptrs(c, pa, t);
+  };
+  CHK(fill(&c->p_Asc, [&](size_t j) { return c->Asc + j * nn; }, B2));
+  CHK(fill(&c->p_A21, [&](size_t j) { return c->Asc + j * nn + h1; }, B2));
+  CHK(fill(&c->p_A12, [&](size_t j) { return c->Asc + j * nn + h1 * n; }, B2));
+  CHK(fill(&c->p_A22, [&](size_t j) { return c->Asc + j * nn + h1 + h1 * n; }, B2));
+  CHK(fill(&c->p_Csc, [&](size_t j) { return c->Csc + j * h1 * h2; }, B2));
+  CHK(fill(&c->p_Dsc, [&](size_t j) { return c->Dsc + j * h1 * h2; }, B2));
+  c->invB = B;
   return SKKR_OK;
 }
 
@@ -382,7 +411,7 @@ static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
 static int run_inverse(skkr_ctx* c, const InvArgs& a, int nb) {
   if (nb <= 0) return SKKR_OK;
   if (a.n <= 128) return launch_gjinv(c, a, nb);
-  if (a.n > 256 || a.mode != 0 || !c->Asc || nb > 2 * c->B || a.n != c->n) return SKKR_ERR_UNSUPPORTED;
+  if (a.n > 256 || a.mode != 0 || !c->Asc || nb > 2 * c->invB || a.n != c->n) return SKKR_ERR_UNSUPPORTED;
   const int n = a.n, h1 = 128, h2 = n - 128;
   const z one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
   {
@@ -632,6 +661,7 @@ const char* skkr_error_string(int code) {
     case SKKR_ERR_SGF_DIVERGED: return "decimation diverged";
     case SKKR_ERR_SGF_ITERMAX: return "decimation exceeded itermax";
     case SKKR_ERR_UNSUPPORTED: return "unsupported size or mode";
+    case SKKR_ERR_STR_NOCONV: return "structure-constant shell sum not converged";
   }
   return "unknown error";
 }
@@ -1199,6 +1229,311 @@ int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
     int code = status_code(hs[j]);
     if (status) status[j] = code;
     if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+  }
+  return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Layer structure constants (gstore.f with bulkgeo)
+// ------------------------------------------------------------------------
+struct skkr_str {
+  skkr_str_config cfg;
+  int nl2 = 0, nlay = 0, N = 0, B = 0, nkam = 0, ndir = 0, nblk = 0;
+  double rlmin = 0.0;
+  kkrh::Lattice lat;
+  skkr_ctx* inv = nullptr;  // batched inversion and products of size N
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // tables
+  double *d_kam_sep = nullptr, *d_dir_sep = nullptr, *d_gau = nullptr, *d_flm = nullptr,
+         *d_yb = nullptr;
+  kkr::cx* d_ycf = nullptr;
+  int *d_kam_blk = nullptr, *d_dir_blk = nullptr, *d_r_off = nullptr, *d_rsh_off = nullptr,
+      *d_rsh_n = nullptr, *d_rsh = nullptr, *d_gsh = nullptr, *d_bmap = nullptr, *d_lof = nullptr;
+  double2 *d_rsorted = nullptr, *d_gsorted = nullptr;
+  int ngsh = 0;
+  // batch buffers
+  double* d_k = nullptr;
+  z *d_alpha = nullptr, *d_blk = nullptr, *d_g = nullptr, *d_g1 = nullptr, *d_gs = nullptr,
+    *d_g2d = nullptr;
+  int* d_status = nullptr;
+  PtrArr p_g, p_g1, p_gs;
+};
+
+namespace {
+template <class T>
+int str_alloc(skkr_str* sc, T** p, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(T) * (count ? count : 1)) != hipSuccess) return SKKR_ERR_NOMEM;
+  sc->allocs.push_back(q);
+  *p = (T*)q;
+  return SKKR_OK;
+}
+template <class T>
+int str_upload(skkr_str* sc, T** p, const std::vector<T>& v) {
+  CHK(str_alloc(sc, p, v.size()));
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return SKKR_OK;
+}
+int str_ptrs(skkr_str* sc, PtrArr* pa, z* base, size_t stride, int n) {
+  std::vector<z*> v(n);
+  for (int j = 0; j < n; ++j) v[j] = base + (size_t)j * stride;
+  CHK(str_alloc(sc, &pa->d, (size_t)n));
+  HIPCHK(hipMemcpy(pa->d, v.data(), sizeof(z*) * n, hipMemcpyHostToDevice));
+  return SKKR_OK;
+}
+constexpr int kStrMaxVec = 4096;  // vectors kept per shell list (cut at a shell end)
+}  // namespace
+
+extern "C" {
+
+int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cvec) {
+  if (!out || !cfg || !cvec) return SKKR_ERR_ARG;
+  *out = nullptr;
+  // nextra >= 1: the blocks of the outer principal layers reach s layers
+  // beyond them (gstore.f:243-276 reads layers i - s .. i + s)
+  if (cfg->lmax < 1 || cfg->s < 1 || cfg->nextra < 1 || !(cfg->eta > 0.0) || cfg->batch < 0)
+    return SKKR_ERR_ARG;
+  if (cfg->lmax > 3) return SKKR_ERR_UNSUPPORTED;
+  const int nl2 = (cfg->lmax + 1) * (cfg->lmax + 1), nlay = (2 * cfg->nextra + 3) * cfg->s;
+  if (nl2 * nlay > 256) return SKKR_ERR_UNSUPPORTED;
+  const double cell = cfg->a1[0] * cfg->a2[1] - cfg->a1[1] * cfg->a2[0];
+  if (!(std::fabs(cell) > 0.0)) return SKKR_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || cfg->device < 0 || cfg->device >= ndev) return SKKR_ERR_HIP;
+  HIPCHK(hipSetDevice(cfg->device));
+  skkr_str* sc = new skkr_str();
+  sc->cfg = *cfg;
+  sc->nl2 = nl2;
+  sc->nlay = nlay;
+  sc->N = nl2 * nlay;
+  sc->B = cfg->batch ? cfg->batch : 256;
+  auto fail = [&](int r) {
+    skkr_str_destroy(sc);
+    return r;
+  };
+  // inversion / product engine of size N
+  skkr_config ic;
+  memset(&ic, 0, sizeof(ic));
+  ic.lk = sc->N;
+  ic.sites_per_pl = 1;
+  ic.n_pl = 1;
+  ic.errmax = 1e-8;
+  ic.itermax_l = ic.itermax_r = 1;
+  ic.precision = SKKR_FP64;
+  ic.device = cfg->device;
+  int r = skkr_create(&sc->inv, &ic);
+  if (r != SKKR_OK) return fail(r);
+  sc->stream = sc->inv->stream;
+  // lattice and the stack of layers (gstore.f:99-122)
+  sc->lat = kkrh::make_lattice(cfg->a1, cfg->a2, 20000);
+  const auto& L = sc->lat;
+  sc->rlmin = std::sqrt(std::fmin(L.b1[0] * L.b1[0] + L.b1[1] * L.b1[1], L.b2[0] * L.b2[0] + L.b2[1] * L.b2[1]));
+  const int s = cfg->s;
+  std::vector<double> cv(3 * (size_t)nlay);
+  double cb[3];
+  for (int d = 0; d < 3; ++d) cb[d] = cvec[3 * s + d] - cvec[d];
+  for (int i = 0; i < s; ++i)
+    for (int d = 0; d < 3; ++d) cv[3 * i + d] = cvec[3 * i + d] - cvec[d];
+  for (int ip = 2; ip <= 2 * cfg->nextra + 3; ++ip)
+    for (int i = 0; i < s; ++i)
+      for (int d = 0; d < 3; ++d) cv[3 * ((ip - 1) * s + i) + d] = cv[3 * i + d] + (ip - 1) * cb[d];
+  // distinct separations of all layer pairs (gstore.f:164-202): block (j, i)
+  // of G holds the pair (i, j) with sep = cvec(j) - cvec(i); Kambe for close
+  // layers, the direct sum (j > i gives blocks (j, i) and (i, j)) otherwise
+  std::vector<double> kam_sep, dir_sep;
+  std::vector<int> kam_blk, dir_blk, bmap((size_t)nlay * nlay, -1);
+  auto same = [](const double* a, const double* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2]; };
+  for (int i = 0; i < nlay; ++i)
+    for (int j = 0; j < nlay; ++j) {
+      const double sp[3] = {cv[3 * j] - cv[3 * i], cv[3 * j + 1] - cv[3 * i + 1], cv[3 * j + 2] - cv[3 * i + 2]};
+      if (std::fabs(sp[2]) * sc->rlmin >= 5.0) {
+        if (j <= i) continue;
+        int u = -1;
+        for (int q = 0; q < (int)dir_blk.size(); ++q)
+          if (same(&dir_sep[3 * q], sp)) u = q;
+        if (u < 0) {
+          u = (int)dir_blk.size();
+          for (int d = 0; d < 3; ++d) dir_sep.push_back(sp[d]);
+          dir_blk.push_back(sc->nblk);
+          sc->nblk += 2;
+        }
+        bmap[(size_t)j * nlay + i] = dir_blk[u];
+        bmap[(size_t)i * nlay + j] = dir_blk[u] + 1;
+      } else {
+        int u = -1;
+        for (int q = 0; q < (int)kam_blk.size(); ++q)
+          if (same(&kam_sep[4 * q + 1], sp)) u = q;
+        if (u < 0) {
+          u = (int)kam_blk.size();
+          kam_sep.push_back(std::sqrt(sp[0] * sp[0] + sp[1] * sp[1] + sp[2] * sp[2]));
+          for (int d = 0; d < 3; ++d) kam_sep.push_back(sp[d]);
+          kam_blk.push_back(sc->nblk);
+          sc->nblk += 1;
+        }
+        bmap[(size_t)j * nlay + i] = kam_blk[u];
+      }
+    }
+  for (int v : bmap)
+    if (v < 0) return fail(SKKR_ERR_ARG);
+  sc->nkam = (int)kam_blk.size();
+  sc->ndir = (int)dir_blk.size();
+  // shells: reciprocal vectors around the origin, real vectors around each
+  // Kambe pair's in-plane offset (vecsrt2d.f, gstore.f:184-188)
+  std::vector<double> gs, rs_all;
+  std::vector<int> gsh, rsh_all, r_off, rsh_off, rsh_n;
+  const double zero2[2] = {0.0, 0.0};
+  kkrh::shells(L.veck, zero2, L.distk, kStrMaxVec, gs, gsh);
+  for (int u = 0; u < sc->nkam; ++u) {
+    std::vector<double> rs;
+    std::vector<int> rsh;
+    const double rho[2] = {kam_sep[4 * u + 1], kam_sep[4 * u + 2]};
+    kkrh::shells(L.rvec, rho, L.distr, kStrMaxVec, rs, rsh);
+    r_off.push_back((int)(rs_all.size() / 2));
+    rsh_off.push_back((int)rsh_all.size());
+    rsh_n.push_back((int)rsh.size());
+    rs_all.insert(rs_all.end(), rs.begin(), rs.end());
+    rsh_all.insert(rsh_all.end(), rsh.begin(), rsh.end());
+  }
+  sc->ngsh = (int)gsh.size();
+  std::vector<double2> g2(gs.size() / 2), r2(rs_all.size() / 2);
+  for (size_t i = 0; i < g2.size(); ++i) g2[i] = make_double2(gs[2 * i], gs[2 * i + 1]);
+  for (size_t i = 0; i < r2.size(); ++i) r2[i] = make_double2(rs_all[2 * i], rs_all[2 * i + 1]);
+  // tables
+  const int l2 = 2 * cfg->lmax;
+  std::vector<double> yb;
+  std::vector<std::complex<double>> ycf;
+  kkrh::ylm_tables(l2, yb, ycf);
+  std::vector<kkr::cx> ycf2(ycf.size());
+  for (size_t i = 0; i < ycf.size(); ++i) ycf2[i] = kkr::cx{ycf[i].real(), ycf[i].imag()};
+  std::vector<int> lof(nl2);
+  for (int l = 0, i = 0; l <= cfg->lmax; ++l)
+    for (int m = -l; m <= l; ++m) lof[i++] = l;
+  if ((r = str_upload(sc, &sc->d_kam_sep, kam_sep)) || (r = str_upload(sc, &sc->d_dir_sep, dir_sep)) ||
+      (r = str_upload(sc, &sc->d_kam_blk, kam_blk)) || (r = str_upload(sc, &sc->d_dir_blk, dir_blk)) ||
+      (r = str_upload(sc, &sc->d_bmap, bmap)) || (r = str_upload(sc, &sc->d_gsorted, g2)) ||
+      (r = str_upload(sc, &sc->d_gsh, gsh)) || (r = str_upload(sc, &sc->d_rsorted, r2)) ||
+      (r = str_upload(sc, &sc->d_r_off, r_off)) || (r = str_upload(sc, &sc->d_rsh_off, rsh_off)) ||
+      (r = str_upload(sc, &sc->d_rsh_n, rsh_n)) || (r = str_upload(sc, &sc->d_rsh, rsh_all)) ||
+      (r = str_upload(sc, &sc->d_gau, kkrh::gaunt_table(cfg->lmax))) ||
+      (r = str_upload(sc, &sc->d_flm, kkrh::kambe_flm(l2))) || (r = str_upload(sc, &sc->d_yb, yb)) ||
+      (r = str_upload(sc, &sc->d_ycf, ycf2)) || (r = str_upload(sc, &sc->d_lof, lof)))
+    return fail(r);
+  // batch buffers
+  const int B = sc->B, N = sc->N;
+  const size_t NN = (size_t)N * N, per = (size_t)nl2 * nl2 * (2 * s + 1) * s * 3;
+  if ((r = str_alloc(sc, &sc->d_k, 2 * (size_t)B)) || (r = str_alloc(sc, &sc->d_alpha, (size_t)(cfg->lmax + 1) * s)) ||
+      (r = str_alloc(sc, &sc->d_blk, (size_t)B * sc->nblk * nl2 * nl2)) ||
+      (r = str_alloc(sc, &sc->d_g, B * NN)) || (r = str_alloc(sc, &sc->d_g1, B * NN)) ||
+      (r = str_alloc(sc, &sc->d_gs, B * NN)) || (r = str_alloc(sc, &sc->d_g2d, B * per)) ||
+      (r = str_alloc(sc, &sc->d_status, (size_t)B)) || (r = str_ptrs(sc, &sc->p_g, sc->d_g, NN, B)) ||
+      (r = str_ptrs(sc, &sc->p_g1, sc->d_g1, NN, B)) || (r = str_ptrs(sc, &sc->p_gs, sc->d_gs, NN, B)) ||
+      (r = ensure_inverse(sc->inv, B)))
+    return fail(r);
+  *out = sc;
+  return SKKR_OK;
+}
+
+void skkr_str_destroy(skkr_str* sc) {
+  if (!sc) return;
+  hipSetDevice(sc->cfg.device);
+  if (sc->stream) hipStreamSynchronize(sc->stream);
+  for (void* p : sc->allocs) hipFree(p);
+  if (sc->inv) skkr_destroy(sc->inv);
+  delete sc;
+}
+
+int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alpha, int nk, const double* k,
+                      double* g2d, int* status) {
+  if (!sc || !alpha || !k || !g2d || nk < 0) return SKKR_ERR_ARG;
+  if (nk == 0) return SKKR_OK;
+  HIPCHK(hipSetDevice(sc->cfg.device));
+  const int lmax = sc->cfg.lmax, s = sc->cfg.s, nl2 = sc->nl2, N = sc->N;
+  const size_t per = (size_t)nl2 * nl2 * (2 * s + 1) * s * 3;
+  HIPCHK(hipMemcpyAsync(sc->d_alpha, alpha, sizeof(z) * (lmax + 1) * s, hipMemcpyHostToDevice, sc->stream));
+  kkr::StrArgs a;
+  memset(&a, 0, sizeof(a));
+  a.e = kkr::cx{e_re, e_im};
+  a.eta = sc->cfg.eta;
+  a.vol = sc->lat.vol;
+  a.nkam = sc->nkam;
+  a.kam_sep = sc->d_kam_sep;
+  a.kam_blk = sc->d_kam_blk;
+  a.rsorted = sc->d_rsorted;
+  a.r_off = sc->d_r_off;
+  a.rsh_off = sc->d_rsh_off;
+  a.rsh_n = sc->d_rsh_n;
+  a.rsh = sc->d_rsh;
+  a.gsorted = sc->d_gsorted;
+  a.gsh = sc->d_gsh;
+  a.ngsh = sc->ngsh;
+  a.gau = sc->d_gau;
+  a.flm = sc->d_flm;
+  a.yb = sc->d_yb;
+  a.ycf = sc->d_ycf;
+  a.ndir = sc->ndir;
+  a.dir_sep = sc->d_dir_sep;
+  a.dir_blk = sc->d_dir_blk;
+  a.blk = sc->d_blk;
+  a.nblk = sc->nblk;
+  a.status = sc->d_status;
+  a.k = sc->d_k;
+  int rc = SKKR_OK;
+  std::vector<int> hs;
+  for (int k0 = 0; k0 < nk; k0 += sc->B) {
+    const int nb = std::min(sc->B, nk - k0);
+    a.nk = nb;
+    HIPCHK(hipMemcpyAsync(sc->d_k, k + 2 * (size_t)k0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, sc->stream));
+    HIPCHK(hipMemsetAsync(sc->d_status, 0, sizeof(int) * nb, sc->stream));
+    if (sc->nkam) {
+      dim3 grid(nb, sc->nkam);
+      if (lmax == 1) kkr::kkr_kamstr_kernel<1><<<grid, 64, 0, sc->stream>>>(a);
+      else if (lmax == 2) kkr::kkr_kamstr_kernel<2><<<grid, 64, 0, sc->stream>>>(a);
+      else kkr::kkr_kamstr_kernel<3><<<grid, 64, 0, sc->stream>>>(a);
+      HIPCHK(hipGetLastError());
+    }
+    if (sc->ndir) {
+      dim3 grid(nb, sc->ndir);
+      if (lmax == 1) kkr::kkr_directg_kernel<1><<<grid, 64, 0, sc->stream>>>(a);
+      else if (lmax == 2) kkr::kkr_directg_kernel<2><<<grid, 64, 0, sc->stream>>>(a);
+      else kkr::kkr_directg_kernel<3><<<grid, 64, 0, sc->stream>>>(a);
+      HIPCHK(hipGetLastError());
+    }
+    {
+      dim3 grid((unsigned)(((long long)N * N + 255) / 256), nb);
+      kkr::kkr_assemble_kernel<<<grid, 256, 0, sc->stream>>>(N, nl2, sc->nlay, s, lmax + 1, sc->d_bmap, sc->d_blk,
+                                                             sc->nblk, sc->d_alpha, sc->d_lof, sc->d_g, sc->d_g1, nb);
+      HIPCHK(hipGetLastError());
+    }
+    InvArgs ia;
+    memset(&ia, 0, sizeof(ia));
+    ia.n = N;
+    ia.lk = N;
+    ia.base = sc->p_g1.d;
+    ia.out = sc->p_g1.d;
+    ia.status = sc->d_status;
+    CHK(run_inverse(sc->inv, ia, nb));
+    CHK(launch_zgemm(sc->inv, N, N, N, sc->p_g.d, N, sc->p_g1.d, N, sc->p_gs.d, N, make_double2(1, 0),
+                     make_double2(0, 0), nb, nullptr));
+    {
+      dim3 grid((unsigned)((per + 255) / 256), nb);
+      kkr::kkr_extract_kernel<<<grid, 256, 0, sc->stream>>>(N, nl2, s, sc->cfg.nextra, sc->d_gs, sc->d_g2d, nb);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipMemcpyAsync(g2d + 2 * per * (size_t)k0, sc->d_g2d, sizeof(z) * per * nb, hipMemcpyDeviceToHost,
+                          sc->stream));
+    hs.resize(nb);
+    HIPCHK(hipMemcpyAsync(hs.data(), sc->d_status, sizeof(int) * nb, hipMemcpyDeviceToHost, sc->stream));
+    HIPCHK(hipStreamSynchronize(sc->stream));
+    for (int j = 0; j < nb; ++j) {
+      int code = SKKR_OK;
+      if (hs[j] & (2 | 4 | 8)) code = SKKR_ERR_STR_NOCONV;  // kamstr G / R, directg
+      else if (hs[j] & 1) code = SKKR_ERR_SINGULAR;
+      if (status) status[k0 + j] = code;
+      if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+    }
   }
   return rc;
 }

@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libskkr_gpu.so")
 SKKR_OK = 0
 ERRORS = {1: "SKKR_ERR_ARG", 2: "SKKR_ERR_HIP", 3: "SKKR_ERR_NOMEM",
           4: "SKKR_ERR_SINGULAR", 5: "SKKR_ERR_SGF_DIVERGED",
-          6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED"}
+          6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED", 8: "SKKR_ERR_STR_NOCONV"}
 FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
 
 # exported symbols (every declaration of include/skkr_gpu.h)
@@ -33,7 +33,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_set_tinv", "skkr_tau_bz", "skkr_pl_blocks", "skkr_gjinv_batch",
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
-           "skkr_tau_bz_phys")
+           "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch")
 
 
 class SkkrError(RuntimeError):
@@ -104,6 +104,10 @@ def lib():
         L.skkr_autotune.argtypes = [V, ctypes.POINTER(Tuning)]
         L.skkr_set_screening.argtypes = [V, I, V, V]
         L.skkr_tau_bz_phys.argtypes = [V, D, D, I, V, V, V, V, V, V, V, ctypes.POINTER(Stats)]
+        L.skkr_str_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(StrConfig), V]
+        L.skkr_str_destroy.argtypes = [V]
+        L.skkr_str_destroy.restype = None
+        L.skkr_gstore_batch.argtypes = [V, D, D, V, I, V, V, V]
         _LIB = L
     return _LIB
 
@@ -114,6 +118,12 @@ def _p(a):
 
 def _c128(a):
     return np.ascontiguousarray(a, np.complex128)
+
+
+class StrConfig(ctypes.Structure):
+    _fields_ = [("lmax", ctypes.c_int), ("s", ctypes.c_int), ("nextra", ctypes.c_int),
+                ("eta", ctypes.c_double), ("a1", ctypes.c_double * 2), ("a2", ctypes.c_double * 2),
+                ("device", ctypes.c_int), ("batch", ctypes.c_int)]
 
 
 class Context:
@@ -288,3 +298,55 @@ class Context:
         status = np.zeros(nb, np.int32)
         lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
         return from_colmajor(tb), status
+
+
+class Structure:
+    """Layer structure constants on the GPU (skkr_str_create ...
+    skkr_str_destroy): the reference's gstore (skkr_kpar_v6/main/gstore.f:6)
+    for the bulk-like geometry, per k-point."""
+
+    def __init__(self, lmax, s, nextra, eta, a1, a2, cvec, device=0, batch=0):
+        self.lmax, self.s, self.nextra = lmax, s, nextra
+        self.nl2 = (lmax + 1) ** 2
+        cfg = StrConfig(lmax, s, nextra, eta, (ctypes.c_double * 2)(*a1), (ctypes.c_double * 2)(*a2),
+                        device, batch)
+        cv = np.ascontiguousarray(cvec, np.float64)
+        if cv.shape != (s + 1, 3):
+            raise ValueError("cvec must be (s+1) x 3")
+        h = ctypes.c_void_p()
+        Context._check(lib().skkr_str_create(ctypes.byref(h), ctypes.byref(cfg), _p(cv)), "skkr_str_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().skkr_str_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def gstore(self, E, alpha, k, check=True):
+        """g2d per k as an array [k, icpl, icount, jj+s, n, m] (the reverse of
+        the reference's g2d(m, n, jj, icount, icpl)); alpha[layer, l]."""
+        al = _c128(alpha)
+        if al.shape != (self.s, self.lmax + 1):
+            raise ValueError("alpha must be s x (lmax+1)")
+        k = np.ascontiguousarray(k, np.float64).reshape(-1, 2)
+        nk = len(k)
+        s, nl2 = self.s, self.nl2
+        out = np.zeros((nk, 3, s, 2 * s + 1, nl2, nl2), np.complex128)
+        st = np.zeros(nk, np.int32)
+        rc = lib().skkr_gstore_batch(self._h, float(np.real(E)), float(np.imag(E)), _p(al), nk,
+                                     _p(k), _p(out), _p(st))
+        if check:
+            Context._check(rc, "skkr_gstore_batch")
+        return out, st

@@ -85,3 +85,26 @@ def test_screening_entry_points_validate_arguments_without_device():
     lib.skkr_tau_bz_phys.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int] + [ctypes.c_void_p] * 8
     assert lib.skkr_tau_bz_phys(None, 0.5, 0.1, 0, *([None] * 8)) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_structure_constant_entry_points_validate_arguments_without_device():
+    from skkr_budapest_amd import gpu
+    lib = ctypes.CDLL(LIB)
+    lib.skkr_str_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.skkr_str_create(None, None, None) == 1                      # SKKR_ERR_ARG
+    h = ctypes.c_void_p()
+    cv = (ctypes.c_double * 9)(0, 0, 0, 1, 0, 1, 2, 0, 2)
+    A = ctypes.c_double * 2
+    for (lmax, s, nextra, eta, code) in ((2, 2, 0, 1.0, 1),   # nextra < 1
+                                         (2, 2, 1, 0.0, 1),   # eta <= 0
+                                         (4, 2, 1, 1.0, 7),   # lmax > 3
+                                         (3, 4, 1, 1.0, 7)):  # nl2 (2 nextra + 3) s > 256
+        cfg = gpu.StrConfig(lmax, s, nextra, eta, A(5.0, 0.0), A(0.0, 5.0), 0, 0)
+        assert lib.skkr_str_create(ctypes.byref(h), ctypes.byref(cfg), cv) == code
+    lib.skkr_gstore_batch.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.skkr_gstore_batch(None, 0.5, 0.05, None, 1, None, None, None) == 1
+    lib.skkr_error_string.restype = ctypes.c_char_p
+    assert lib.skkr_error_string(8) == b"structure-constant shell sum not converged"

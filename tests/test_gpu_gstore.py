@@ -1,0 +1,102 @@
+"""GPU parity of the layer structure constants (skkr_gstore_batch through the
+C-ABI) against the reference's own gstore (golden vectors of
+tests/golden/gstore.npz) and against the oracle (oracle/kkrstr_oracle.c, itself
+pinned to those vectors) on other k, energies and l cutoffs.
+
+Tolerance: 1e-10 relative to the largest entry.  The GPU inverts
+1 - alpha G with pivoting restricted to 16-row blocks and a 2x2 block split
+above 128 rows; the reference pivots over whole rows (scrinv.f).  Both agree to
+round-off times the conditioning of 1 - alpha G (~1e-12 here)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "gstore.npz"), allow_pickle=False)
+LMAX = int(GOLD["lmax"])
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+def _geo(geo):
+    p = geo + "_"
+    return dict(a1=GOLD[p + "a1"], a2=GOLD[p + "a2"], s=int(GOLD[p + "s"]), nextra=int(GOLD[p + "nextra"]),
+                cvec=GOLD[p + "cvec"], E=complex(GOLD[p + "E"]), eta=float(GOLD[p + "eta"]),
+                al=GOLD[p + "alpha_l"], ar=GOLD[p + "alpha_r"], ks=GOLD[p + "ks"],
+                gl=GOLD[p + "g2d_L"], gr=GOLD[p + "g2d_R"])
+
+
+@pytest.mark.parametrize("geo", ["nb110", "sq2"])
+def test_gstore_matches_reference_golden(gpu, geo):
+    G = gpu
+    g = _geo(geo)
+    print(f"gstore golden {geo}", flush=True)
+    with G.Structure(LMAX, g["s"], g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        out, stat = st.gstore(g["E"], g["al"], g["ks"])
+        assert (stat == 0).all()
+        for ik in range(len(g["ks"])):
+            assert rel(out[ik], g["gl"][ik]) < TOL
+        out, stat = st.gstore(g["E"], g["ar"], g["ks"])
+        for ik in range(len(g["ks"])):
+            assert rel(out[ik], g["gr"][ik]) < TOL
+
+
+def test_gstore_matches_oracle_on_other_k_and_energy(gpu):
+    G = gpu
+    g = _geo("nb110")
+    rng = np.random.default_rng(11)
+    ks = rng.uniform(-1.0, 1.0, size=(5, 2))
+    E = complex(0.3, 0.02)
+    L = oracle.Lattice(g["a1"], g["a2"])
+    print("gstore nb110 random k, E=0.3+0.02i", flush=True)
+    with G.Structure(LMAX, g["s"], g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        out, stat = st.gstore(E, g["al"], ks)
+    assert (stat == 0).all()
+    for ik, k in enumerate(ks):
+        ref = L.gstore(LMAX, g["s"], g["nextra"], g["cvec"], g["al"], E, k, g["eta"])
+        assert rel(out[ik], ref) < TOL
+
+
+def test_gstore_batches_are_bitwise_independent(gpu):
+    """Results per k do not depend on how the k-points are batched."""
+    G = gpu
+    g = _geo("sq2")
+    rng = np.random.default_rng(12)
+    ks = rng.uniform(-0.6, 0.6, size=(7, 2))
+    print("gstore sq2 batch 3 vs 7", flush=True)
+    with G.Structure(LMAX, g["s"], g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"], batch=3) as st:
+        a, _ = st.gstore(g["E"], g["al"], ks)
+    with G.Structure(LMAX, g["s"], g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        b, _ = st.gstore(g["E"], g["al"], ks)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("lmax", [1, 3])
+def test_gstore_other_l_cutoffs_match_oracle(gpu, lmax):
+    """lmax 1 (matrix of all layer pairs within 128 rows: one-kernel inversion)
+    and lmax 3 (160 rows: 2x2 block inversion) on the square cell."""
+    G = gpu
+    g = _geo("sq2")
+    al = np.concatenate([g["al"], 0.5 * g["al"][:, -1:]], axis=1) if lmax == 3 else g["al"][:, :2]
+    ks = np.array([[0.12, -0.33], [0.5, 0.05]])
+    L = oracle.Lattice(g["a1"], g["a2"])
+    print(f"gstore sq2 lmax={lmax}", flush=True)
+    with G.Structure(lmax, g["s"], g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        out, stat = st.gstore(g["E"], al, ks)
+    assert (stat == 0).all()
+    for ik, k in enumerate(ks):
+        ref = L.gstore(lmax, g["s"], g["nextra"], g["cvec"], al, g["E"], k, g["eta"])
+        assert rel(out[ik], ref) < TOL
