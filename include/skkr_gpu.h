@@ -252,6 +252,20 @@ void skkr_str_destroy(skkr_str *sc);
 int skkr_gstore_batch(skkr_str *sc, double e_re, double e_im, const double *alpha,
                       int nk, const double *k, double *g2d, int *status);
 
+/* Use the layer producer for the structure constants of the k-loop instead
+ * of skkr_set_structure's Fourier sum (sc = NULL switches back): every
+ * skkr_tau_bz / skkr_tau_bz_phys call then computes, per k at its energy,
+ * the screened constants of gstore (side 'L', bulkgeo: cpacoord.f:205) and
+ * builds the blocks as packer does (packer.f:36-97): M00, M01, M10 of the
+ * middle principal layer for the decimation and the sweeps (tau2d.f:55-56),
+ * and the self-energies Y_1 = M10 g_L M01(plane 0), X_P = M01 g_R M10(plane 2)
+ * (dugo.f:71-75, 98, 106).  alpha: complex (lmax+1) x s as for
+ * skkr_gstore_batch, at the energy of the next tau_bz calls (alphamat.f is
+ * energy dependent).  Requires lk = (lmax+1)^2 (the non-relativistic lm
+ * basis, one spin channel per context as nrsp-scf runs them) and
+ * sites_per_pl = s.  A producer serves one context at a time. */
+int skkr_set_structure_kkr(skkr_ctx *ctx, skkr_str *sc, const double *alpha);
+
 #ifdef __cplusplus
 }
 #endif

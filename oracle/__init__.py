@@ -318,3 +318,64 @@ class Lattice:
         if st:
             raise RuntimeError(f"orc_gstore status {st}")
         return out.reshape(3, s, 2 * s + 1, nl2, nl2)
+
+
+def kkr_blocks(g2d, s, nl2):
+    """packer.f:36-97 (equal ninprc, irel = 0) on the g2d planes of one k
+    (array [icpl, icount, jj+s, n, m] as Lattice.gstore returns it): M00, M01,
+    M10 of plane 1, M01 of plane 0 and M10 of plane 2 (the outer factors of
+    the lead self-energies, dugo.f:71-75)."""
+    n = s * nl2
+
+    def blk(p, i, jj):
+        return g2d[p, i, jj + s].T  # g2d(m, n, jj, i, p) as [m, n]
+
+    def plane(p):
+        m00, m01, m10 = (np.zeros((n, n), np.complex128) for _ in range(3))
+        for i in range(s):
+            r = slice(i * nl2, (i + 1) * nl2)
+            for jp in range(s):
+                c = slice(jp * nl2, (jp + 1) * nl2)
+                m00[r, c] = -blk(p, i, jp - i)
+                if jp <= i:
+                    m01[r, c] = -blk(p, i, jp + s - i)
+                if jp >= i:
+                    m10[r, c] = -blk(p, i, jp - s - i)
+        return m00, m01, m10
+
+    m00, m01, m10 = plane(1)
+    return m00, m01, m10, plane(0)[1], plane(2)[2]
+
+
+def kkr_tau_bz(lattice, lmax, s, nextra, cvec, alpha, E, eta, tinv, tinv_l, tinv_r, ks, ws,
+               itermax=50, errmax=1e-8):
+    """k-loop of nrsp-scf/cpacoord.f:147-311 with bulkgeo on the oracle:
+    gstore (side 'L') -> packer -> dugo('L'), dugo('R') with the outer factors
+    of planes 0 and 2 -> tau2d; BZ sum of the weighted site-diagonal tau.
+    tinv: (P*s, nl2, nl2); tinv_l, tinv_r: (s, nl2, nl2).  Returns
+    (tau_bz [sites, nl2, nl2], per-k decimation iterations [k, 2])."""
+    nl2 = (lmax + 1) ** 2
+    P = tinv.shape[0] // s
+    acc = np.zeros((P * s, nl2, nl2), np.complex128)
+    its = []
+    for k, w in zip(ks, ws):
+        g2d = lattice.gstore(lmax, s, nextra, cvec, alpha, E, k, eta)
+        m00, m01, m10, mtl, mtr = kkr_blocks(g2d, s, nl2)
+        tl = m00.copy()
+        tr = m00.copy()
+        for i in range(s):
+            sl = slice(i * nl2, (i + 1) * nl2)
+            tl[sl, sl] += tinv_l[i]
+            tr[sl, sl] += tinv_r[i]
+        gl, itl, st1 = surfgfd(m10, tl, m01, itermax, errmax)
+        gr, itr, st2 = surfgfd(m01, tr, m10, itermax, errmax)
+        if st1 or st2:
+            raise RuntimeError(f"surfgfd status {st1}, {st2} at k={k}")
+        y = m10 @ gl @ mtl
+        x = m01 @ gr @ mtr
+        tau, st = tau2d(m00, m01, m10, tinv, P, x, y, w)
+        if st:
+            raise RuntimeError(f"tau2d status {st} at k={k}")
+        acc += tau
+        its.append((itl, itr))
+    return acc, np.array(its)

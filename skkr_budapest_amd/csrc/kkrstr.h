@@ -737,4 +737,43 @@ __global__ __launch_bounds__(256) void kkr_extract_kernel(int N, int nl2, int s,
   g2d[(size_t)kb * per + idx] = gs[(size_t)kb * N * N + (size_t)(i * nl2 + m) + (size_t)(j * nl2 + n) * N];
 }
 
+// PL blocks of the k-loop from the screened constants of one k (packer.f:36-97
+// with equal ninprc, irel = 0): site i, j of the middle principal layer
+// (plane 1) give M00, M01 (j <= i) and M10 (j >= i); M01 of plane 0 and M10
+// of plane 2 are the outer factors of the lead self-energies (dugo.f:71-75,
+// 98, 106).  Row (i, k1) of an n x n block is i * nl2 + k1.
+__global__ __launch_bounds__(256) void kkr_pack_kernel(int N, int nl2, int s, int nextra,
+                                                       const double2* __restrict__ gs, int nb, long long nn,
+                                                       double2* M00, double2* M01, double2* M10,
+                                                       double2* MtL, double2* MtR) {
+  const int n = nl2 * s;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int kb = blockIdx.y;
+  if (idx >= (long long)n * n || kb >= nb) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  const int i = r / nl2, k1 = r % nl2, jp = c / nl2, k2 = c % nl2;
+  const double2* g = gs + (size_t)kb * N * N;
+  auto at = [&](int la, int lb) {
+    const double2 v = g[(size_t)(la * nl2 + k1) + (size_t)(lb * nl2 + k2) * N];
+    return make_double2(-v.x, -v.y);
+  };
+  const double2 zero = make_double2(0.0, 0.0);
+  const int p0 = nextra * s, p1 = p0 + s, p2 = p1 + s;
+  const size_t o = (size_t)kb * nn + idx;
+  M00[o] = at(p1 + i, p1 + jp);
+  M01[o] = (jp <= i) ? at(p1 + i, p2 + jp) : zero;
+  M10[o] = (jp >= i) ? at(p1 + i, p0 + jp) : zero;
+  MtL[o] = (jp <= i) ? at(p0 + i, p1 + jp) : zero;
+  MtR[o] = (jp >= i) ? at(p2 + i, p1 + jp) : zero;
+}
+
+// producer status of a k -> k-loop status bits (8: not converged, 1: pivot)
+__global__ void kkr_status_kernel(int nb, const int* __restrict__ st, int* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const int v = st[j];
+  if (v & (2 | 4 | 8)) out[j] |= 8;
+  if (v & 1) out[j] |= 1;
+}
+
 }  // namespace kkr

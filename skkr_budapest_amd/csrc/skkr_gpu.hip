@@ -119,6 +119,13 @@ struct skkr_ctx {
   z* phk = nullptr;      // per-k per-site per-l traces of a batch
   int phk_cap = 0;
   std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by batch size nb
+  // structure constants from the layer producer instead of the Fourier sum
+  skkr_str* str = nullptr;
+  z* strAlpha = nullptr;          // (lmax+1) x s screening parameters
+  z *MtL = nullptr, *MtR = nullptr;  // outer factors of the lead self-energies
+  PtrArr p_mt;                    // per chain: MtL (left), MtR (right)
+  int mt_cap = 0;
+  double e_re = 0.0, e_im = 0.0;  // energy of the current tau_bz call
 };
 
 // ------------------------------------------------------------------------
@@ -489,8 +496,8 @@ static int run_fourier(skkr_ctx* c, int nk) {
 // [ay; by] = [alpha; beta] y, then [ay; by] beta and [ay; by] alpha with the
 // updates of eps, x and the next [alpha; beta] fused into the GEMM epilogues.
 // Leaves g_s = x^-1 in Yinv and the self-energy ml g_s mr in p_sigma when
-// want_sigma.
-static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
+// want_sigma (ml g_s mt with the outer factors mt when given).
+static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const PtrArr* mt = nullptr) {
   const int n = c->n, B2 = 2 * c->B;
   dec_init_kernel<<<B2, 256, 0, c->stream>>>(n, c->cfg.lk, c->p_ml.d, c->p_mr.d, c->p_m0.d,
                                              add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
@@ -551,7 +558,7 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma) {
     // T = ml g_s (into AYBY as n x n), sigma = T mr
     CHK(launch_zgemm(c, n, n, n, c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero, B2,
                      c->active));
-    CHK(launch_zgemm(c, n, n, n, c->p_AYBY.d, n, c->p_mr.d, n, c->p_sigma.d, n, one, zero, B2,
+    CHK(launch_zgemm(c, n, n, n, c->p_AYBY.d, n, mt ? mt->d : c->p_mr.d, n, c->p_sigma.d, n, one, zero, B2,
                      c->active));
   }
   return SKKR_OK;
@@ -640,6 +647,7 @@ static int run_sweeps(skkr_ctx* c, int nb) {
 }
 
 static int status_code(int bits) {
+  if (bits & 8) return SKKR_ERR_STR_NOCONV;
   if (bits & 4) return SKKR_ERR_SGF_ITERMAX;
   if (bits & 2) return SKKR_ERR_SGF_DIVERGED;
   if (bits & 1) return SKKR_ERR_SINGULAR;
@@ -777,15 +785,31 @@ static int auto_batch(skkr_ctx* c, int nk) {
 
 // The k-loop of one energy; with phys, also the physical representation:
 // tau_bz_phys += phystau of this call's BZ sum, taukdiag = per-k per-l traces.
+static int str_blocks(skkr_ctx* c, int nb);
+
+// outer self-energy factors of the chains (layer producer only)
+static int ensure_mt(skkr_ctx* c) {
+  if (c->mt_cap >= c->B) return SKKR_OK;
+  const size_t nn = c->nn, Bs = c->B;
+  CHK(dalloc_t(c, &c->MtL, Bs * nn));
+  CHK(dalloc_t(c, &c->MtR, Bs * nn));
+  std::vector<z*> v(2 * Bs);
+  for (size_t j = 0; j < 2 * Bs; ++j) v[j] = j < Bs ? c->MtL + j * nn : c->MtR + (j - Bs) * nn;
+  CHK(make_ptrs(c, &c->p_mt, v));
+  c->mt_cap = c->B;
+  return SKKR_OK;
+}
+
 static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, double* tau_bz,
                       double* tr_tau_k, int* iters_k, int* status_k, skkr_stats* st,
                       bool phys, double* tau_bz_phys, double* taukdiag) {
   if (!c || nk < 0 || (nk > 0 && (!k || !w)) || !tau_bz) return SKKR_ERR_ARG;
-  if (!c->have_structure || !c->have_tinv) return SKKR_ERR_ARG;
+  if (!(c->have_structure || c->str) || !c->have_tinv) return SKKR_ERR_ARG;
   if (phys && (!c->have_screening || !tau_bz_phys)) return SKKR_ERR_ARG;
   HIPCHK(hipSetDevice(c->cfg.device));
   auto t0 = std::chrono::steady_clock::now();
   CHK(ensure_batch(c, auto_batch(c, std::max(nk, 1))));
+  if (c->str) CHK(ensure_mt(c));
   // the chain layout on the device (left chains 0..B-1, right chains B..2B-1)
   // follows the allocated batch, which an earlier call may have made larger
   const int B = c->B;
@@ -821,9 +845,12 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
     HIPCHK(hipMemsetAsync(c->status_sw, 0, B * sizeof(int), c->stream));
     hipEventRecord(ev[0], c->stream);
-    CHK(run_fourier(c, nb));
+    if (c->str)
+      CHK(str_blocks(c, nb));
+    else
+      CHK(run_fourier(c, nb));
     hipEventRecord(ev[1], c->stream);
-    CHK(run_decimation(c, true, true));
+    CHK(run_decimation(c, true, true, c->str ? &c->p_mt : nullptr));
     hipEventRecord(ev[2], c->stream);
     CHK(run_sweeps(c, nb));
     hipEventRecord(ev[3], c->stream);
@@ -940,8 +967,9 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
 int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
                 const double* w, double* tau_bz, double* tr_tau_k, int* iters_k,
                 int* status_k, skkr_stats* st) {
-  (void)e_re;
-  (void)e_im;
+  if (!c) return SKKR_ERR_ARG;
+  c->e_re = e_re;
+  c->e_im = e_im;
   return tau_bz_run(c, nk, k, w, tau_bz, tr_tau_k, iters_k, status_k, st, false, nullptr,
                     nullptr);
 }
@@ -949,8 +977,9 @@ int skkr_tau_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
 int skkr_tau_bz_phys(skkr_ctx* c, double e_re, double e_im, int nk, const double* k,
                      const double* w, double* tau_bz, double* tau_bz_phys, double* taukdiag,
                      int* iters_k, int* status_k, skkr_stats* st) {
-  (void)e_re;
-  (void)e_im;
+  if (!c) return SKKR_ERR_ARG;
+  c->e_re = e_re;
+  c->e_im = e_im;
   return tau_bz_run(c, nk, k, w, tau_bz, nullptr, iters_k, status_k, st, true, tau_bz_phys,
                     taukdiag);
 }
@@ -1445,16 +1474,17 @@ void skkr_str_destroy(skkr_str* sc) {
   delete sc;
 }
 
-int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alpha, int nk, const double* k,
-                      double* g2d, int* status) {
-  if (!sc || !alpha || !k || !g2d || nk < 0) return SKKR_ERR_ARG;
-  if (nk == 0) return SKKR_OK;
-  HIPCHK(hipSetDevice(sc->cfg.device));
+}  // extern "C"
+
+// screened constants of nb <= sc->B k-points (device k) into sc->d_gs, on
+// stream; status per k in sc->d_status
+static int str_run(skkr_str* sc, hipStream_t stream, double e_re, double e_im, const z* alpha, const double* dk,
+                   int nb) {
   const int lmax = sc->cfg.lmax, s = sc->cfg.s, nl2 = sc->nl2, N = sc->N;
-  const size_t per = (size_t)nl2 * nl2 * (2 * s + 1) * s * 3;
-  HIPCHK(hipMemcpyAsync(sc->d_alpha, alpha, sizeof(z) * (lmax + 1) * s, hipMemcpyHostToDevice, sc->stream));
   kkr::StrArgs a;
   memset(&a, 0, sizeof(a));
+  a.nk = nb;
+  a.k = dk;
   a.e = kkr::cx{e_re, e_im};
   a.eta = sc->cfg.eta;
   a.vol = sc->lat.vol;
@@ -1479,44 +1509,98 @@ int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alph
   a.blk = sc->d_blk;
   a.nblk = sc->nblk;
   a.status = sc->d_status;
-  a.k = sc->d_k;
+  HIPCHK(hipMemsetAsync(sc->d_status, 0, sizeof(int) * nb, stream));
+  if (sc->nkam) {
+    dim3 grid(nb, sc->nkam);
+    if (lmax == 1) kkr::kkr_kamstr_kernel<1><<<grid, 64, 0, stream>>>(a);
+    else if (lmax == 2) kkr::kkr_kamstr_kernel<2><<<grid, 64, 0, stream>>>(a);
+    else kkr::kkr_kamstr_kernel<3><<<grid, 64, 0, stream>>>(a);
+    HIPCHK(hipGetLastError());
+  }
+  if (sc->ndir) {
+    dim3 grid(nb, sc->ndir);
+    if (lmax == 1) kkr::kkr_directg_kernel<1><<<grid, 64, 0, stream>>>(a);
+    else if (lmax == 2) kkr::kkr_directg_kernel<2><<<grid, 64, 0, stream>>>(a);
+    else kkr::kkr_directg_kernel<3><<<grid, 64, 0, stream>>>(a);
+    HIPCHK(hipGetLastError());
+  }
+  {
+    dim3 grid((unsigned)(((long long)N * N + 255) / 256), nb);
+    kkr::kkr_assemble_kernel<<<grid, 256, 0, stream>>>(N, nl2, sc->nlay, s, lmax + 1, sc->d_bmap, sc->d_blk,
+                                                       sc->nblk, alpha, sc->d_lof, sc->d_g, sc->d_g1, nb);
+    HIPCHK(hipGetLastError());
+  }
+  // the inversion and the product run on the engine's launch paths, on this stream
+  hipStream_t own = sc->inv->stream;
+  sc->inv->stream = stream;
+  InvArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = N;
+  ia.lk = N;
+  ia.base = sc->p_g1.d;
+  ia.out = sc->p_g1.d;
+  ia.status = sc->d_status;
+  int r = run_inverse(sc->inv, ia, nb);
+  if (r == SKKR_OK)
+    r = launch_zgemm(sc->inv, N, N, N, sc->p_g.d, N, sc->p_g1.d, N, sc->p_gs.d, N, make_double2(1, 0),
+                     make_double2(0, 0), nb, nullptr);
+  sc->inv->stream = own;
+  return r;
+}
+
+// M00, M01, M10 and the self-energy factors of nb k-points of the k-loop
+// from the layer producer (replaces the Fourier sum)
+static int str_blocks(skkr_ctx* c, int nb) {
+  skkr_str* sc = c->str;
+  for (int k0 = 0; k0 < nb; k0 += sc->B) {
+    const int nbb = std::min(sc->B, nb - k0);
+    CHK(str_run(sc, c->stream, c->e_re, c->e_im, c->strAlpha, c->dk + 2 * (size_t)k0, nbb));
+    const size_t nn = c->nn, o = (size_t)k0 * nn;
+    dim3 grid((unsigned)((nn + 255) / 256), nbb);
+    kkr::kkr_pack_kernel<<<grid, 256, 0, c->stream>>>(sc->N, sc->nl2, sc->cfg.s, sc->cfg.nextra, sc->d_gs, nbb,
+                                                       (long long)nn, c->M00 + o, c->M01 + o, c->M10 + o,
+                                                       c->MtL + o, c->MtR + o);
+    HIPCHK(hipGetLastError());
+    kkr::kkr_status_kernel<<<(nbb + 255) / 256, 256, 0, c->stream>>>(nbb, sc->d_status, c->status_sw + k0);
+    HIPCHK(hipGetLastError());
+    c->launches += 2;
+  }
+  return SKKR_OK;
+}
+
+extern "C" {
+
+int skkr_set_structure_kkr(skkr_ctx* c, skkr_str* sc, const double* alpha) {
+  if (!c) return SKKR_ERR_ARG;
+  if (!sc) {
+    c->str = nullptr;
+    return SKKR_OK;
+  }
+  if (!alpha || c->cfg.lk != sc->nl2 || c->cfg.sites_per_pl != sc->cfg.s || c->cfg.device != sc->cfg.device)
+    return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t na = (size_t)(sc->cfg.lmax + 1) * sc->cfg.s;
+  if (!c->strAlpha) CHK(dalloc_t(c, &c->strAlpha, na));
+  HIPCHK(hipMemcpyAsync(c->strAlpha, alpha, sizeof(z) * na, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->str = sc;
+  return SKKR_OK;
+}
+
+int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alpha, int nk, const double* k,
+                      double* g2d, int* status) {
+  if (!sc || !alpha || !k || !g2d || nk < 0) return SKKR_ERR_ARG;
+  if (nk == 0) return SKKR_OK;
+  HIPCHK(hipSetDevice(sc->cfg.device));
+  const int lmax = sc->cfg.lmax, s = sc->cfg.s, nl2 = sc->nl2, N = sc->N;
+  const size_t per = (size_t)nl2 * nl2 * (2 * s + 1) * s * 3;
+  HIPCHK(hipMemcpyAsync(sc->d_alpha, alpha, sizeof(z) * (lmax + 1) * s, hipMemcpyHostToDevice, sc->stream));
   int rc = SKKR_OK;
   std::vector<int> hs;
   for (int k0 = 0; k0 < nk; k0 += sc->B) {
     const int nb = std::min(sc->B, nk - k0);
-    a.nk = nb;
     HIPCHK(hipMemcpyAsync(sc->d_k, k + 2 * (size_t)k0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, sc->stream));
-    HIPCHK(hipMemsetAsync(sc->d_status, 0, sizeof(int) * nb, sc->stream));
-    if (sc->nkam) {
-      dim3 grid(nb, sc->nkam);
-      if (lmax == 1) kkr::kkr_kamstr_kernel<1><<<grid, 64, 0, sc->stream>>>(a);
-      else if (lmax == 2) kkr::kkr_kamstr_kernel<2><<<grid, 64, 0, sc->stream>>>(a);
-      else kkr::kkr_kamstr_kernel<3><<<grid, 64, 0, sc->stream>>>(a);
-      HIPCHK(hipGetLastError());
-    }
-    if (sc->ndir) {
-      dim3 grid(nb, sc->ndir);
-      if (lmax == 1) kkr::kkr_directg_kernel<1><<<grid, 64, 0, sc->stream>>>(a);
-      else if (lmax == 2) kkr::kkr_directg_kernel<2><<<grid, 64, 0, sc->stream>>>(a);
-      else kkr::kkr_directg_kernel<3><<<grid, 64, 0, sc->stream>>>(a);
-      HIPCHK(hipGetLastError());
-    }
-    {
-      dim3 grid((unsigned)(((long long)N * N + 255) / 256), nb);
-      kkr::kkr_assemble_kernel<<<grid, 256, 0, sc->stream>>>(N, nl2, sc->nlay, s, lmax + 1, sc->d_bmap, sc->d_blk,
-                                                             sc->nblk, sc->d_alpha, sc->d_lof, sc->d_g, sc->d_g1, nb);
-      HIPCHK(hipGetLastError());
-    }
-    InvArgs ia;
-    memset(&ia, 0, sizeof(ia));
-    ia.n = N;
-    ia.lk = N;
-    ia.base = sc->p_g1.d;
-    ia.out = sc->p_g1.d;
-    ia.status = sc->d_status;
-    CHK(run_inverse(sc->inv, ia, nb));
-    CHK(launch_zgemm(sc->inv, N, N, N, sc->p_g.d, N, sc->p_g1.d, N, sc->p_gs.d, N, make_double2(1, 0),
-                     make_double2(0, 0), nb, nullptr));
+    CHK(str_run(sc, sc->stream, e_re, e_im, sc->d_alpha, sc->d_k, nb));
     {
       dim3 grid((unsigned)((per + 255) / 256), nb);
       kkr::kkr_extract_kernel<<<grid, 256, 0, sc->stream>>>(N, nl2, s, sc->cfg.nextra, sc->d_gs, sc->d_g2d, nb);

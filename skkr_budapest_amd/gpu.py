@@ -33,7 +33,8 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_set_tinv", "skkr_tau_bz", "skkr_pl_blocks", "skkr_gjinv_batch",
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
-           "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch")
+           "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
+           "skkr_set_structure_kkr")
 
 
 class SkkrError(RuntimeError):
@@ -108,6 +109,7 @@ def lib():
         L.skkr_str_destroy.argtypes = [V]
         L.skkr_str_destroy.restype = None
         L.skkr_gstore_batch.argtypes = [V, D, D, V, I, V, V, V]
+        L.skkr_set_structure_kkr.argtypes = [V, V, V]
         _LIB = L
     return _LIB
 
@@ -170,6 +172,20 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_structure_kkr(self, structure, alpha):
+        """Take the structure constants of the k-loop from a ``Structure``
+        (the layer producer) with screening alpha[layer, l] at the energy of
+        the next tau_bz calls; ``structure=None`` returns to the Fourier sum."""
+        if structure is None:
+            self._check(lib().skkr_set_structure_kkr(self._h, None, None), "skkr_set_structure_kkr")
+            self._str = None
+            return
+        al = _c128(alpha)
+        if al.shape != (structure.s, structure.lmax + 1):
+            raise ValueError("alpha must be s x (lmax+1)")
+        self._check(lib().skkr_set_structure_kkr(self._h, structure._h, _p(al)), "skkr_set_structure_kkr")
+        self._str = structure  # keeps the producer alive while bound
 
     def set_option(self, key, value):
         self._check(lib().skkr_set_option(self._h, key.encode(), int(value)),

@@ -100,3 +100,47 @@ def test_gstore_other_l_cutoffs_match_oracle(gpu, lmax):
     for ik, k in enumerate(ks):
         ref = L.gstore(lmax, g["s"], g["nextra"], g["cvec"], al, g["E"], k, g["eta"])
         assert rel(out[ik], ref) < TOL
+
+
+def _kkr_system(E):
+    """The Nb(110)-like stack with square-well screening (V = 3 Ry) and the
+    screened inverse t-matrix of an attractive well (V = -1 Ry),
+    t_scr^-1 = t^-1 - alpha^-1, diagonal in lm; 3 interface PLs."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_gstore_golden import square_well_alpha
+    g = _geo("nb110")
+    rmt = 0.3481599413 * 4.667470441
+    al = np.array([square_well_alpha(E, rmt) for _ in range(g["s"])])
+    tv = 1.0 / square_well_alpha(E, rmt, v=-1.0) - 1.0 / al[0]
+    lof = [l for l in range(LMAX + 1) for _ in range(2 * l + 1)]
+    ti = np.array([np.diag([tv[l] for l in lof]) for _ in range(g["s"])])
+    return g, al, ti
+
+
+def test_kloop_with_layer_structure_constants_matches_oracle(gpu):
+    """skkr_tau_bz with the structure constants of the layer producer
+    (skkr_set_structure_kkr) against the oracle's k-loop of the same geometry
+    (gstore -> packer -> dugo with the plane-0/plane-2 outer factors ->
+    tau2d); decimation iteration counts equal."""
+    G = gpu
+    E = complex(0.5, 0.05)
+    g, al, ti = _kkr_system(E)
+    s, P, nl2 = g["s"], 3, (LMAX + 1) ** 2
+    tinv = np.concatenate([ti] * P)
+    rng = np.random.default_rng(13)
+    ks = rng.uniform(-np.pi / 4.667, np.pi / 4.667, size=(6, 2))
+    ws = np.full(len(ks), 1.0 / len(ks))
+    L = oracle.Lattice(g["a1"], g["a2"])
+    ref, its = oracle.kkr_tau_bz(L, LMAX, s, g["nextra"], g["cvec"], al, E, g["eta"], tinv, ti, ti, ks, ws)
+    print("k-loop with layer structure constants, nb110, 6 k", flush=True)
+    with G.Structure(LMAX, s, g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        with G.Context(lk=nl2, sites_per_pl=s, n_pl=P) as ctx:
+            ctx.set_tinv(tinv, ti, ti)
+            ctx.set_structure_kkr(st, al)
+            r = ctx.tau_bz(E, ks, ws)
+            np.testing.assert_array_equal(r["iters"], its)
+            assert rel(r["tau_bz"], ref) < TOL
+            ctx.set_structure_kkr(None, None)
+            with pytest.raises(G.SkkrError):  # no Fourier structure set either
+                ctx.tau_bz(E, ks, ws)
