@@ -8,6 +8,11 @@ Default workload: the relativistic layer system with lk=32, 4 sites per
 principal layer (n=128), 20 interface PLs, 25 in-plane vectors, one energy
 E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points).
 
+--kkr runs the k-loop on the reference's own structure constants instead of
+the synthetic Fourier sum: the Nb(110) stack of sample/02-layer-scf (lmax 2,
+4 layers per principal layer, 3 interface PLs), Kambe/direct sums and the
+screening per k on the GPU, E = 0.5+0.05i, 64x64 mesh of the reciprocal cell.
+
 --config 2 / 3 run a 16-point contour instead (a step is the whole contour):
 config 2 the same system with a 64x64 mesh per energy, config 3 the BdG
 system (lk=64, n=256, 12 PLs) with 256 ... 16384 k per energy, geometric in
@@ -156,6 +161,50 @@ def mesh_bench(args, cfg, sy):
     print(json.dumps(out))
 
 
+def kkr_bench(args):
+    """k-loop with structure constants from the layer producer (no CPU leg:
+    the oracle's per-k gstore takes ~0.3 s)."""
+    from skkr_budapest_amd import gpu
+    from skkr_budapest_amd import kkrsystems as K
+    E = complex(0.5, 0.05)
+    wl = K.nb110_workload(E)
+    st = wl["structure"]
+    k, w = K.reciprocal_mesh(st["a1"], st["a2"], 64)
+    nk = len(w)
+    print(f"bench: kkr Nb(110) nk={nk} E={E}", file=sys.stderr, flush=True)
+    with gpu.Structure(**st) as sc, gpu.Context(lk=(st["lmax"] + 1) ** 2, sites_per_pl=st["s"],
+                                                 n_pl=wl["P"]) as ctx:
+        ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
+        ctx.set_structure_kkr(sc, wl["alpha"])
+        for i in range(args.warmup):
+            ctx.tau_bz(E, k, w)
+        times, stats = [], []
+        for i in range(args.steps):
+            t0 = time.perf_counter()
+            r = ctx.tau_bz(E, k, w, timing_detail=True)
+            times.append(time.perf_counter() - t0)
+            stats.append(r["stats"])
+            print(f"bench: step {i} {times[-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr, flush=True)
+    ms = 1e3 * float(np.mean(times))
+    stl = stats[-1]
+    out = {
+        "metric": METRIC, "value": nk / (ms * 1e-3), "unit": "k-points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic t^-1 (square wells), reference-form structure constants",
+        "config": {"workload": f"Nb(110) stack on the layer producer: lmax={st['lmax']} s={st['s']} "
+                               f"nextra={st['nextra']} P={wl['P']} n={st['s'] * (st['lmax'] + 1) ** 2} "
+                               f"E={E} 64x64 k",
+                   "precision": "fp64", "nk": nk, "mean_iter_l": stl["mean_iter_l"],
+                   "mean_iter_r": stl["mean_iter_r"],
+                   "phase_ms": {"structure_constants": stl["ms_fourier"], "decimation": stl["ms_decimation"],
+                                "sweeps": stl["ms_sweeps"], "accumulate": stl["ms_accumulate"]}},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +212,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kkr", action="store_true",
+                    help="k-loop on the layer structure-constant producer (Nb(110) stack)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -173,6 +224,8 @@ def main():
     from skkr_budapest_amd import gpu
     from skkr_budapest_amd import systems as S
 
+    if args.kkr:
+        return kkr_bench(args)
     cfg = S.CONFIGS[args.config]
     sy = S.make_system(cfg)
     if args.config in (2, 3):

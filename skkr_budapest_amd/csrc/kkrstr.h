@@ -290,9 +290,10 @@ struct StrArgs {
   const int* rsh_off;      // first shell of the pair in rsh
   const int* rsh_n;        // shells of the pair
   const int* rsh;          // vectors per shell
+  const int* r_nvec;       // vectors of the pair in rsorted
   const double2* gsorted;  // reciprocal vectors sorted around the origin
   const int* gsh;          // vectors per shell
-  int ngsh;                // shells
+  int ngsh, ngvec;         // shells, vectors
   const double* gau;       // Gaunt table (lm1, lm2, lm)
   const double* flm;       // F(lm) of kamstr.f:79-91
   const double* yb;        // ylmc.f b(l, m, k), (L2+1)^3
@@ -308,17 +309,29 @@ struct StrArgs {
                            // (1 is the inversion's zero pivot)
 };
 
+// sum over the 64 lanes of a wavefront, returned to every lane
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // One wavefront per (k, Kambe pair): D_lm of kamstr.f:202-361 (G sum,
 // formula 35 or 50), :363-373 (R sum, formula 47) and the Gaunt contraction
-// :374-392.  Lane = lm of the l <= 2 LMAX coefficient.
+// :374-392.  Vectors are taken 64 at a time: first lane t evaluates the
+// special functions of vector t of the chunk into LDS, then every lane (one
+// per lm of l <= 2 LMAX) adds the chunk's vectors in order to its D_lm and
+// the shells' stop tests run as they complete.
 template <int LMAX>
 __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
   constexpr int L2 = 2 * LMAX, NLM2 = (L2 + 1) * (L2 + 1), NL2 = (LMAX + 1) * (LMAX + 1);
   static_assert(NLM2 <= 64, "lanes cover the l <= 2 lmax coefficients");
   const double pi = 3.14159265358979323846, ctol = 1.0e-4, tin = 1.0e-10;
-  __shared__ double sabs[64];
+  __shared__ cx s_f0[64][L2 + 1], s_f1[64][L2 + 1], s_cp[64][L2 + 1], s_cm[64][L2 + 1];
+  __shared__ cx s_e1[64], s_e2[64];
+  __shared__ double s_d0[64], s_d1[64];
+  __shared__ cx s_cs[L2 + 1];
   __shared__ cx sd[64];
-  __shared__ cx s_cs[L2 + 1], s_cp[L2 + 1], s_cm[L2 + 1], s_gam[L2 + 1], s_del[L2 + 1];
   const int lane = threadIdx.x;
   const int kb = blockIdx.x, pr = blockIdx.y;
   if (kb >= a.nk || pr >= a.nkam) return;  // uniform per work-group
@@ -337,7 +350,7 @@ __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
   double sx = a.kam_sep[4 * pr + 1], sy = a.kam_sep[4 * pr + 2], sz = a.kam_sep[4 * pr + 3];
   double s = a.kam_sep[4 * pr + 0];
   cx dlm1 = mk(0.0), dlm2 = mk(0.0);
-  if (s < ctol && ln == 0) {
+  if (s < ctol && ln == 0 && act) {
     const cx sa = csqrt(alpha);
     const cx c = mul_i(cexp(-alpha) - cerf(sa)) - 1.0 / sqrt(pi) / sa;
     dlm1 = -kappa * (cexp(alpha) * c - cx{0.0, 1.0}) / 2.0 / sqrt(pi);
@@ -347,28 +360,27 @@ __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
     s = sqrt(sx * sx + sy * sy + sz * sz);
   }
   const bool f35 = (s > ctol && fabs(sz) > ctol), f50 = (s < ctol);
-  {
+  if (lane == 0) {
     const cx c = kappa * sz;
     cx t = mk(1.0);
-    if (lane == 0) {
-      s_cs[0] = t;
+    s_cs[0] = t;
 #pragma unroll
-      for (int i = 1; i <= L2; ++i) {
-        t = t * c;
-        s_cs[i] = t;
-      }
+    for (int i = 1; i <= L2; ++i) {
+      t = t * c;
+      s_cs[i] = t;
     }
   }
   int status = 0;
-  // ---- reciprocal-space sum, shell by shell
+  // ---- reciprocal-space sum (s_f1 = Delta_n for formula 35, s_f0 = Gamma for 50)
   {
-    int no = 0, is;
+    int ish = 0, left = a.gsh[0];
     double tst = 1.0e30;
-    bool conv = false;
-    for (is = 1; is <= a.ngsh; ++is) {
-      const int nv = a.gsh[is - 1];
-      for (int iv = 0; iv < nv; ++iv, ++no) {
-        const double2 g = a.gsorted[no];
+    bool conv = false, done = false;
+    for (int v0 = 0; v0 < a.ngvec && !done; v0 += 64) {
+      const int nv = min(64, a.ngvec - v0);
+      __syncthreads();
+      if (lane < nv) {
+        const double2 g = a.gsorted[v0 + lane];
         const double gkx = g.x + kx, gky = g.y + ky;
         double km = gkx * gkx + gky * gky;
         const cx kp = csqrt(e - mk(km));
@@ -376,48 +388,47 @@ __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
         cx ex2 = mk(1.0);
         if (km > 1.0e-10) ex2 = cx{gkx / km, gky / km};
         const cx xc = cx{-1.0, -1.2246467991473532e-16} * (e - mk(km * km)) * w / 2.0;
-        // per-vector quantities shared by all lanes, staged in LDS so that
-        // the lane-dependent series below index them freely
-        {
-          cx gam[L2 + 1], del[L2 + 1];
-          incgamma<L2>(xc, gam);
-          if (f35) {
-            const cx zq = kp * sz;
-            delkam<L2>(xc, zq * zq, del);
-          }
-          const cx r = kp / kappa, c = r * r;
-          cx t = 1.0 / (r * r * r);
-          const cx q = km / kappa;
-          cx u = mk(1.0);
-          __syncthreads();
-          if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i <= L2; ++i) {
-              t = t * c;
-              s_cp[i] = t;
-              s_cm[i] = u;
-              u = u * q;
-              s_gam[i] = gam[i];
-              s_del[i] = f35 ? del[i] : mk(0.0);
-            }
-          }
-          __syncthreads();
-        }
+        cx f[L2 + 1];
         if (f35) {
+          const cx zq = kp * sz;
+          delkam<L2>(xc, zq * zq, f);
+#pragma unroll
+          for (int i = 0; i <= L2; ++i) s_f1[lane][i] = f[i];
           double sn, csn;
           sincos(gkx * sx + gky * sy, &sn, &csn);
-          const cx ex1 = cx{csn, sn} * cs1;
-          cx ex = cipow(ex2, l + 1) * ex1;
-          for (int mi = -l; mi <= m; ++mi) ex = ex / ex2;
-          cx csum = mk(0.0);
+          s_e1[lane] = cx{csn, sn} * cs1;
+        } else {
+          incgamma<L2>(xc, f);
+#pragma unroll
+          for (int i = 0; i <= L2; ++i) s_f0[lane][i] = f[i];
+          s_e1[lane] = cs1;
+        }
+        const cx r = kp / kappa, c = r * r, q = km / kappa;
+        cx t = 1.0 / (r * r * r), u = mk(1.0);
+#pragma unroll
+        for (int i = 0; i <= L2; ++i) {
+          t = t * c;
+          s_cp[lane][i] = t;
+          s_cm[lane][i] = u;
+          u = u * q;
+        }
+        s_e2[lane] = ex2;
+      }
+      __syncthreads();
+      for (int v = 0; v < nv; ++v) {
+        const cx ex2 = s_e2[v];
+        cx ex = cipow(ex2, l + 1) * s_e1[v];
+        for (int mi = -l; mi <= m; ++mi) ex = ex / ex2;
+        cx csum = mk(0.0);
+        if (f35) {
           const int ipar = (l - mm) % 2;
           for (int n = 0; n <= l - mm; ++n) {
-            const cx an = s_cp[n] * s_del[n];
+            const cx an = s_cp[v][n] * s_f1[v][n];
             cx b = mk(0.0);
             const int kend = (n + n < l - mm) ? n + n : l - mm;
             for (int k = n + (n + ipar) % 2; k <= kend; k += 2) {
               int i = n + n - k, j = l - k;
-              const cx cc = s_cs[i] * s_cm[j] / kFac[i] / kFac[k - n];
+              const cx cc = s_cs[i] * s_cm[v][j] / kFac[i] / kFac[k - n];
               i = (l - mm - k) / 2;
               j = (l + mm - k) / 2;
               b = b + cc / kFac[i] / kFac[j];
@@ -426,90 +437,105 @@ __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
           }
           if (act) dlm1 = dlm1 + ex * fl * csum * ipow_i(1 - m);
         } else if (f50) {
-          cx ex = cipow(ex2, l + 1) * cs1;
-          for (int mi = -l; mi <= m; ++mi) ex = ex / ex2;
-          cx csum = mk(0.0);
           if ((l - mm) % 2 == 0) {
             for (int n = 0; n <= (l - mm) / 2; ++n) {
-              const int i = l - 2 * n;
-              const cx an = s_cp[n] * s_cm[i] / kFac[n] * s_gam[n];
+              const cx an = s_cp[v][n] * s_cm[v][l - 2 * n] / kFac[n] * s_f0[v][n];
               csum = csum + an / kFac[(l - mm - 2 * n) / 2] / kFac[(l + mm - 2 * n) / 2];
             }
           }
           if (act) dlm1 = dlm1 + ex * csum * fl * ipow_i(1 - m);
         }
+        if (--left == 0) {  // shell ish complete: stop test of kamstr.f:355-357
+          const double d = wave_sum(act ? cabs(dlm1) : 0.0);
+          if (fabs(tst - d) < tin && ish + 1 >= 3) {
+            conv = done = true;
+            break;
+          }
+          tst = d;
+          if (++ish >= a.ngsh) {
+            done = true;
+            break;
+          }
+          left = a.gsh[ish];
+        }
       }
-      sabs[lane] = act ? cabs(dlm1) : 0.0;
-      __syncthreads();
-      double d = 0.0;
-      for (int i = 0; i < NLM2; ++i) d += sabs[i];
-      __syncthreads();
-      if (fabs(tst - d) < tin && is >= 3) {
-        conv = true;
-        break;
-      }
-      tst = d;
     }
     if (!conv) status |= 2;
   }
-  // ---- real-space sum around (sx, sy), shell by shell
+  // ---- real-space sum around (sx, sy): s_f0 = I_l, s_e1 = ex1 cs2 / a,
+  // s_e2 = a = -kappa rs / 2, s_cm[.][0] = e^{i phi}, s_d0 = cos theta,
+  // s_d1 = sin theta (negative: vector skipped)
   if (!status) {
     const double2* rv = a.rsorted + a.r_off[pr];
     const int* rsh = a.rsh + a.rsh_off[pr];
-    const int nsh = a.rsh_n[pr];
-    // ylmc.f coefficients of this lane's (l, |m|)
+    const int nsh = a.rsh_n[pr], nvec = a.r_nvec[pr];
     double yb[L2 + 1];
 #pragma unroll
     for (int k = 0; k <= L2; ++k) yb[k] = a.yb[(l * (L2 + 1) + mm) * (L2 + 1) + k];
     const cx ycf = a.ycf[ln];
-    int no = 0, is;
+    int ish = 0, left = nsh > 0 ? rsh[0] : 0;
     double tst = 1.0e30;
-    bool conv = false;
-    for (is = 1; is <= nsh; ++is) {
-      const int nv = rsh[is - 1];
-      for (int iv = 0; iv < nv; ++iv, ++no) {
-        const double2 r = rv[no];
+    bool conv = false, done = (nsh == 0);
+    for (int v0 = 0; v0 < nvec && !done; v0 += 64) {
+      const int nv = min(64, nvec - v0);
+      __syncthreads();
+      if (lane < nv) {
+        const double2 r = rv[v0 + lane];
         const double rk = r.x * kx + r.y * ky;
         double sn, csn;
         sincos(rk, &sn, &csn);
         const cx ex1 = cx{csn, -sn};
         const double rsx = r.x + sx, rsy = r.y + sy, rsz = sz;
         const double rs = sqrt(rsx * rsx + rsy * rsy + rsz * rsz);
-        if (rs < 1.0e-5) continue;
-        // Y_lm of (rsx, rsy, rsz) as ylmc.f:96-121
-        const double cth = rsz / rs, dxy = sqrt(rsx * rsx + rsy * rsy);
-        const cx ephi = (dxy < 1.0e-6) ? mk(1.0) : cx{rsx / dxy, rsy / dxy};
-        const double px = sqrt(1.0 - cth * cth) + 1.0e-30;
-        double t = 0.0, pk = 1.0;
+        if (rs < 1.0e-5) {
+          s_d1[lane] = -1.0;
+        } else {
+          const double cth = rsz / rs, dxy = sqrt(rsx * rsx + rsy * rsy);
+          s_cm[lane][0] = (dxy < 1.0e-6) ? mk(1.0) : cx{rsx / dxy, rsy / dxy};
+          s_d0[lane] = cth;
+          s_d1[lane] = sqrt(1.0 - cth * cth) + 1.0e-30;
+          cx f[L2 + 1];
+          ilkam<L2>(alpha, rs, kappa, f);
 #pragma unroll
-        for (int k = 0; k <= L2; ++k) {
-          if (k > l) break;
-          t += yb[k] * pk;
-          pk *= cth;
+          for (int i = 0; i <= L2; ++i) s_f0[lane][i] = f[i];
+          const cx av = -kappa * rs / 2.0;
+          s_e2[lane] = av;
+          s_e1[lane] = ex1 * cs2 / av;
         }
-        const cx ylm = ycf * pow(px, (double)mm) * t * cipow(ephi, m);
-        cx intil[L2 + 1];
-        ilkam<L2>(alpha, rs, kappa, intil);
-        const cx av = -kappa * rs / 2.0;
-        cx b = ex1 * cs2 / av;
-        cx c = mk(0.0);
+      }
+      __syncthreads();
+      for (int v = 0; v < nv; ++v) {
+        const double px = s_d1[v];
+        if (px >= 0.0) {
+          const double cth = s_d0[v];
+          double t = 0.0, pk = 1.0;
 #pragma unroll
-        for (int i = 0; i <= L2; ++i) {
-          b = b * av;
-          if (i == l) c = intil[i] * b;
+          for (int k = 0; k <= L2; ++k) {
+            if (k > l) break;
+            t += yb[k] * pk;
+            pk *= cth;
+          }
+          const cx ylm = ycf * pow(px, (double)mm) * t * cipow(s_cm[v][0], m);
+          const cx av = s_e2[v];
+          cx b = s_e1[v];
+          for (int i = 0; i <= l; ++i) b = b * av;
+          const cx c = s_f0[v][l] * b;
+          if (act) dlm2 = dlm2 + c * conj(ylm);
         }
-        if (act) dlm2 = dlm2 + c * conj(ylm);
+        if (--left == 0) {
+          const double d = wave_sum(act ? cabs(dlm2) : 0.0);
+          if (fabs(tst - d) < tin && ish + 1 >= 3) {
+            conv = done = true;
+            break;
+          }
+          tst = d;
+          if (++ish >= nsh) {
+            done = true;
+            break;
+          }
+          left = rsh[ish];
+        }
       }
-      sabs[lane] = act ? cabs(dlm2) : 0.0;
-      __syncthreads();
-      double d = 0.0;
-      for (int i = 0; i < NLM2; ++i) d += sabs[i];
-      __syncthreads();
-      if (fabs(tst - d) < tin && is >= 3) {
-        conv = true;
-        break;
-      }
-      tst = d;
     }
     if (!conv) status |= 4;
   }
@@ -518,6 +544,7 @@ __global__ __launch_bounds__(64) void kkr_kamstr_kernel(StrArgs a) {
     return;
   }
   // ---- Gaunt contraction alm(lm1, lm2) = sum_lm gau (D1 + D2)
+  __syncthreads();
   sd[lane] = dlm1 + dlm2;
   __syncthreads();
   double2* out = a.blk + ((size_t)kb * a.nblk + a.kam_blk[pr]) * (NL2 * NL2);
@@ -575,16 +602,19 @@ __device__ __forceinline__ void sphrm(cx* ylm, cx ct, cx st, cx cf) {
   }
 }
 
-// One wavefront per (k, distant pair): gp, gm of directg.f:52-130.  Lanes own
-// entries e = lmj + NL2 * lmi of gp(lmj, lmi) and gm(lmj, lmi).  The stop
-// test of directg.f:125 reads, at entry (lmi, lmj), gp(lmi, lmj) as it stands
-// inside the last vector's double loop: already updated when lmj <= lmi.
+// One wavefront per (k, distant pair): gp, gm of directg.f:52-130.  Lane t
+// first evaluates Y_lm(theta_k, phi_k) and the two phase factors of vector t
+// of a 64-vector chunk into LDS; then lanes own entries e = lmj + NL2 * lmi
+// of gp(lmj, lmi), gm(lmj, lmi) and add the chunk's vectors in order.  The
+// stop test of directg.f:125 reads, at entry (lmi, lmj), gp(lmi, lmj) as it
+// stands inside the last vector's double loop: updated already when
+// lmj <= lmi.
 template <int LMAX>
 __global__ __launch_bounds__(64) void kkr_directg_kernel(StrArgs a) {
   constexpr int NL2 = (LMAX + 1) * (LMAX + 1), NE = NL2 * NL2, PER = (NE + 63) / 64;
   const double pi = 3.14159265358979323846, dthr = 1.0e-12;
-  __shared__ cx y1s[NL2], y2s[NL2], y3s[NL2], y4s[NL2];
-  __shared__ double s_term[64], s_sum[64];
+  __shared__ cx s_ylm[64][NL2];
+  __shared__ cx s_fp[64], s_fm[64];
   const int lane = threadIdx.x;
   const int kb = blockIdx.x, pr = blockIdx.y;
   if (kb >= a.nk || pr >= a.ndir) return;
@@ -592,16 +622,41 @@ __global__ __launch_bounds__(64) void kkr_directg_kernel(StrArgs a) {
   const double s1 = a.dir_sep[3 * pr], s2 = a.dir_sep[3 * pr + 1], s3 = a.dir_sep[3 * pr + 2];
   const cx kappa2 = a.e, kappa = csqrt(kappa2);
   const double f1 = 8.0 * pi * pi / a.vol;
+  // per entry: Y indices and the exact unit factors of directg.f:88-104
+  // (ylm1 = sl il Y_lmi, ylm2 = il sm Y_lmmm, ylm3 = il sm Y_lmi, ylm4 = sl il Y_lmmm)
+  int yi[PER], yj[PER];
+  cx p1[PER], p2[PER], p3[PER], p4[PER];
+  bool ok[PER], upd[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e2 = lane + 64 * q;
+    ok[q] = e2 < NE;
+    const int lmj = ok[q] ? e2 % NL2 : 0, lmi = ok[q] ? e2 / NL2 : 0;
+    int li = 0, lj = 0;
+    while ((li + 1) * (li + 1) <= lmi) ++li;
+    while ((lj + 1) * (lj + 1) <= lmj) ++lj;
+    const int mi = lmi - li * (li + 1), mj = lmj - lj * (lj + 1);
+    const double sli = (li % 2) ? -1.0 : 1.0, slj = (lj % 2) ? -1.0 : 1.0;
+    const double smi = (mi % 2) ? -1.0 : 1.0, smj = (mj % 2) ? -1.0 : 1.0;
+    yi[q] = lmi;
+    yj[q] = lmj - mj - mj;
+    p1[q] = sli * ipow_i(li);
+    p2[q] = ipow_i(lj) * smj;
+    p3[q] = ipow_i(li) * smi;
+    p4[q] = slj * ipow_i(lj);
+    upd[q] = (e2 / NL2) <= (e2 % NL2);  // the stop test's read of entry (row e%NL2, col e/NL2)
+  }
   cx gp[PER], gm[PER], gpo[PER], gmo[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) gp[q] = gm[q] = gpo[q] = gmo[q] = mk(0.0);
-  int no = 0, i1;
-  bool conv = false;
-  for (i1 = 1; i1 <= a.ngsh; ++i1) {
-    double term = 0.0;
-    const int nv = a.gsh[i1 - 1];
-    for (int iv = 0; iv < nv; ++iv, ++no) {
-      const double2 g = a.gsorted[no];
+  int ish = 0, left = a.gsh[0];
+  double term = 0.0;
+  bool conv = false, done = false;
+  for (int v0 = 0; v0 < a.ngvec && !done; v0 += 64) {
+    const int nv = min(64, a.ngvec - v0);
+    __syncthreads();
+    if (lane < nv) {
+      const double2 g = a.gsorted[v0 + lane];
       const double gkx = akx + g.x, gky = aky + g.y;
       const double gk2 = gkx * gkx + gky * gky, gk = sqrt(gk2);
       const cx gkz = csqrt(kappa2 - mk(gk2));
@@ -610,40 +665,21 @@ __global__ __launch_bounds__(64) void kkr_directg_kernel(StrArgs a) {
       const cx cth = gkz / kappa, sth = mk(gk) / kappa;
       cx ylm[NL2];
       sphrm<LMAX>(ylm, cth, sth, phi);
-      // ylm1..4 of directg.f:88-104 into LDS (static indices, one writer each)
-      __syncthreads();
-      {
-        int lm = 0;
-        cx il = cx{0.0, -1.0};
-        double sl = -1.0, sm = -1.0;
 #pragma unroll
-        for (int l = 0; l <= LMAX; ++l) {
-          sl = -sl;
-          il = mul_i(il);
-#pragma unroll
-          for (int m = -l; m <= l; ++m) {
-            sm = -sm;
-            const int lmmm = lm - m - m;
-            if (lane == lm) {
-              y1s[lm] = sl * il * ylm[lm];
-              y2s[lm] = il * sm * ylm[lmmm];
-              y3s[lm] = il * sm * ylm[lm];
-              y4s[lm] = sl * il * ylm[lmmm];
-            }
-            ++lm;
-          }
-        }
-      }
-      __syncthreads();
+      for (int j = 0; j < NL2; ++j) s_ylm[lane][j] = ylm[j];
       const cx argp = mul_i(mk(gkx * s1 + gky * s2) + gkz * s3);
       const cx argm = mul_i(mk(-gkx * s1 - gky * s2) + gkz * s3);
-      const cx fp = cexp(argp) * f1 / gkz, fm = cexp(argm) * f1 / gkz;
+      s_fp[lane] = cexp(argp) * f1 / gkz;
+      s_fm[lane] = cexp(argm) * f1 / gkz;
+    }
+    __syncthreads();
+    for (int v = 0; v < nv; ++v) {
+      const cx fp = s_fp[v], fm = s_fm[v];
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        const int e2 = lane + 64 * q;
-        if (e2 < NE) {
-          const int lmj = e2 % NL2, lmi = e2 / NL2;
-          const cx ctp = y1s[lmi] * fp * y2s[lmj], ctm = y3s[lmi] * fm * y4s[lmj];
+        if (ok[q]) {
+          const cx ya = s_ylm[v][yi[q]], yb = s_ylm[v][yj[q]];
+          const cx ctp = (p1[q] * ya) * fp * (p2[q] * yb), ctm = (p3[q] * ya) * fm * (p4[q] * yb);
           gpo[q] = gp[q];
           gmo[q] = gm[q];
           gp[q] = gp[q] + cx{ctp.i, -ctp.r};  // ctp * (-i)
@@ -651,30 +687,23 @@ __global__ __launch_bounds__(64) void kkr_directg_kernel(StrArgs a) {
           term += cabs(ctp) + cabs(ctm);
         }
       }
-    }
-    // sum of the stop test (entry (a, b) = gp(a, b) stored at e = a + NL2 b is
-    // read at iteration (lmi = a, lmj = b): updated already iff b <= a)
-    double sum = 0.0;
+      if (--left == 0) {  // shell ish complete
+        double sum = 0.0;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e2 = lane + 64 * q;
-      if (e2 < NE) {
-        const int ra = e2 % NL2, cb = e2 / NL2;
-        sum += (cb <= ra) ? cabs(gp[q]) + cabs(gm[q]) : cabs(gpo[q]) + cabs(gmo[q]);
+        for (int q = 0; q < PER; ++q)
+          if (ok[q]) sum += upd[q] ? cabs(gp[q]) + cabs(gm[q]) : cabs(gpo[q]) + cabs(gmo[q]);
+        const double tt = wave_sum(term), ss = wave_sum(sum);
+        if (tt / ss < dthr && ish >= 1) {
+          conv = done = true;
+          break;
+        }
+        term = 0.0;
+        if (++ish >= a.ngsh) {
+          done = true;
+          break;
+        }
+        left = a.gsh[ish];
       }
-    }
-    s_term[lane] = term;
-    s_sum[lane] = sum;
-    __syncthreads();
-    double tt = 0.0, ss = 0.0;
-    for (int i = 0; i < 64; ++i) {
-      tt += s_term[i];
-      ss += s_sum[i];
-    }
-    __syncthreads();
-    if (tt / ss < dthr && i1 > 1) {
-      conv = true;
-      break;
     }
   }
   if (!conv) {

@@ -1280,9 +1280,10 @@ struct skkr_str {
          *d_yb = nullptr;
   kkr::cx* d_ycf = nullptr;
   int *d_kam_blk = nullptr, *d_dir_blk = nullptr, *d_r_off = nullptr, *d_rsh_off = nullptr,
-      *d_rsh_n = nullptr, *d_rsh = nullptr, *d_gsh = nullptr, *d_bmap = nullptr, *d_lof = nullptr;
+      *d_rsh_n = nullptr, *d_rsh = nullptr, *d_gsh = nullptr, *d_bmap = nullptr, *d_lof = nullptr,
+      *d_r_nvec = nullptr;
   double2 *d_rsorted = nullptr, *d_gsorted = nullptr;
-  int ngsh = 0;
+  int ngsh = 0, ngvec = 0;
   // batch buffers
   double* d_k = nullptr;
   z *d_alpha = nullptr, *d_blk = nullptr, *d_g = nullptr, *d_g1 = nullptr, *d_gs = nullptr,
@@ -1412,7 +1413,7 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
   // shells: reciprocal vectors around the origin, real vectors around each
   // Kambe pair's in-plane offset (vecsrt2d.f, gstore.f:184-188)
   std::vector<double> gs, rs_all;
-  std::vector<int> gsh, rsh_all, r_off, rsh_off, rsh_n;
+  std::vector<int> gsh, rsh_all, r_off, rsh_off, rsh_n, r_nvec;
   const double zero2[2] = {0.0, 0.0};
   kkrh::shells(L.veck, zero2, L.distk, kStrMaxVec, gs, gsh);
   for (int u = 0; u < sc->nkam; ++u) {
@@ -1423,10 +1424,13 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
     r_off.push_back((int)(rs_all.size() / 2));
     rsh_off.push_back((int)rsh_all.size());
     rsh_n.push_back((int)rsh.size());
+    r_nvec.push_back((int)(rs.size() / 2));
     rs_all.insert(rs_all.end(), rs.begin(), rs.end());
     rsh_all.insert(rsh_all.end(), rsh.begin(), rsh.end());
   }
   sc->ngsh = (int)gsh.size();
+  sc->ngvec = (int)(gs.size() / 2);
+  if (sc->ngsh == 0) return fail(SKKR_ERR_ARG);
   std::vector<double2> g2(gs.size() / 2), r2(rs_all.size() / 2);
   for (size_t i = 0; i < g2.size(); ++i) g2[i] = make_double2(gs[2 * i], gs[2 * i + 1]);
   for (size_t i = 0; i < r2.size(); ++i) r2[i] = make_double2(rs_all[2 * i], rs_all[2 * i + 1]);
@@ -1446,6 +1450,7 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
       (r = str_upload(sc, &sc->d_gsh, gsh)) || (r = str_upload(sc, &sc->d_rsorted, r2)) ||
       (r = str_upload(sc, &sc->d_r_off, r_off)) || (r = str_upload(sc, &sc->d_rsh_off, rsh_off)) ||
       (r = str_upload(sc, &sc->d_rsh_n, rsh_n)) || (r = str_upload(sc, &sc->d_rsh, rsh_all)) ||
+      (r = str_upload(sc, &sc->d_r_nvec, r_nvec)) ||
       (r = str_upload(sc, &sc->d_gau, kkrh::gaunt_table(cfg->lmax))) ||
       (r = str_upload(sc, &sc->d_flm, kkrh::kambe_flm(l2))) || (r = str_upload(sc, &sc->d_yb, yb)) ||
       (r = str_upload(sc, &sc->d_ycf, ycf2)) || (r = str_upload(sc, &sc->d_lof, lof)))
@@ -1496,9 +1501,11 @@ static int str_run(skkr_str* sc, hipStream_t stream, double e_re, double e_im, c
   a.rsh_off = sc->d_rsh_off;
   a.rsh_n = sc->d_rsh_n;
   a.rsh = sc->d_rsh;
+  a.r_nvec = sc->d_r_nvec;
   a.gsorted = sc->d_gsorted;
   a.gsh = sc->d_gsh;
   a.ngsh = sc->ngsh;
+  a.ngvec = sc->ngvec;
   a.gau = sc->d_gau;
   a.flm = sc->d_flm;
   a.yb = sc->d_yb;

@@ -7,7 +7,7 @@ module skkr_gpu
 
   integer(c_int), parameter :: SKKR_OK = 0, SKKR_ERR_ARG = 1, SKKR_ERR_HIP = 2, &
        SKKR_ERR_NOMEM = 3, SKKR_ERR_SINGULAR = 4, SKKR_ERR_SGF_DIVERGED = 5, &
-       SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7
+       SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7, SKKR_ERR_STR_NOCONV = 8
   integer(c_int), parameter :: SKKR_FP64 = 0, SKKR_MIXED_FP32 = 1, SKKR_MIXED_BF16 = 2
 
   type, bind(C) :: skkr_config
@@ -29,6 +29,14 @@ module skkr_gpu
      character(kind=c_char) :: dom_name(32)
      real(c_double) :: dom_bytes
   end type skkr_stats
+
+  ! layer structure-constant producer (gstore.f with bulkgeo)
+  type, bind(C) :: skkr_str_config
+     integer(c_int) :: lmax, s, nextra
+     real(c_double) :: eta
+     real(c_double) :: a1(2), a2(2)
+     integer(c_int) :: device, batch
+  end type skkr_str_config
 
   interface
      integer(c_int) function skkr_create(ctx, cfg) bind(C, name="skkr_create")
@@ -90,5 +98,39 @@ module skkr_gpu
        complex(c_double_complex), intent(inout) :: tau_bz(*), tau_bz_phys(*)
        type(c_ptr), value :: taukdiag, iters_k, status_k, stats
      end function skkr_tau_bz_phys
+
+     ! producer of the screened layer structure constants (gstore.f:6)
+     integer(c_int) function skkr_str_create(sc, cfg, cvec) bind(C, name="skkr_str_create")
+       import :: c_int, c_ptr, c_double, skkr_str_config
+       type(c_ptr), intent(out) :: sc
+       type(skkr_str_config), intent(in) :: cfg
+       real(c_double), intent(in) :: cvec(*)
+     end function skkr_str_create
+
+     subroutine skkr_str_destroy(sc) bind(C, name="skkr_str_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: sc
+     end subroutine skkr_str_destroy
+
+     ! g2d(nl2, nl2, -s:s, s, 0:2) per k, as gstore returns it
+     integer(c_int) function skkr_gstore_batch(sc, e_re, e_im, alpha, nk, k, g2d, status_k) &
+          bind(C, name="skkr_gstore_batch")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: sc
+       real(c_double), value :: e_re, e_im
+       complex(c_double_complex), intent(in) :: alpha(*)
+       integer(c_int), value :: nk
+       real(c_double), intent(in) :: k(*)
+       complex(c_double_complex), intent(out) :: g2d(*)
+       type(c_ptr), value :: status_k
+     end function skkr_gstore_batch
+
+     ! take the k-loop's structure constants from the producer
+     integer(c_int) function skkr_set_structure_kkr(ctx, sc, alpha) &
+          bind(C, name="skkr_set_structure_kkr")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx, sc
+       complex(c_double_complex), intent(in) :: alpha(*)
+     end function skkr_set_structure_kkr
   end interface
 end module skkr_gpu
