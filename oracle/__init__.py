@@ -379,3 +379,46 @@ def kkr_tau_bz(lattice, lmax, s, nextra, cvec, alpha, E, eta, tinv, tinv_l, tinv
         acc += tau
         its.append((itl, itr))
     return acc, np.array(its)
+
+
+# ---------------------------------------------------------------------------
+# Coherent-potential solver (cpa_oracle.c): tata, cpasol, cpacor of
+# skkr_kpar_v6/nrsp-scf/fullcpa.f.
+
+def _cpa_lib():
+    L = lib()
+    if not hasattr(L, "_cpa_ready"):
+        P = ctypes.c_void_p
+        i, d = ctypes.c_int, ctypes.c_double
+        L.orc_tata.argtypes = [d, i, P, P, P]
+        L.orc_cpasol.argtypes = [d, i, P, P, P, P, P, P, P]
+        L.orc_cpacor.argtypes = [i, i, P, P, P, P, P, P, P, P]
+        for f in (L.orc_tata, L.orc_cpasol, L.orc_cpacor):
+            f.restype = ctypes.c_int
+        L._cpa_ready = True
+    return L
+
+
+def tata(conc, ma, mb):
+    """Initial coherent t^-1 of one site: mb (c mb + (1-c) ma)^-1 ma."""
+    n = ma.shape[0]
+    mc = np.zeros((n, n), np.complex128)
+    a, b, c = _cm(ma), _cm(mb), _cm(mc)
+    _cpa_lib().orc_tata(float(conc), n, _p(c), _p(a), _p(b))
+    return _uncm(c)
+
+
+def cpacor(conc, mc, ma, mb, tauc):
+    """One CPA update over the sites (arrays [site, n, n]): returns the new
+    coherent t^-1, tau_A, tau_B and the error (largest error3)."""
+    nsite, n = ma.shape[0], ma.shape[1]
+    c = _cm(mc)
+    a, b, t = _cm(ma), _cm(mb), _cm(tauc)
+    ta, tb = np.zeros_like(a), np.zeros_like(a)
+    cc = np.ascontiguousarray(conc, np.float64)
+    err = ctypes.c_double(0.0)
+    st = _cpa_lib().orc_cpacor(nsite, n, _p(cc), _p(c), _p(a), _p(b), _p(t), _p(ta), _p(tb),
+                               ctypes.byref(err))
+    if st:
+        raise RuntimeError("orc_cpacor: singular matrix")
+    return _uncm(c), _uncm(ta), _uncm(tb), err.value
