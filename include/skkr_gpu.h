@@ -266,6 +266,39 @@ int skkr_gstore_batch(skkr_str *sc, double e_re, double e_im, const double *alph
  * sites_per_pl = s.  A producer serves one context at a time. */
 int skkr_set_structure_kkr(skkr_ctx *ctx, skkr_str *sc, const double *alpha);
 
+/* ------------------------------------------------------------------------
+ * CPA loop of one energy: nrsp-scf/cpacoord.f:107-345 around the k-loop,
+ * with the solver of nrsp-scf/fullcpa.f (rsp-scf/fullcpa.f is the same in
+ * the kappa-mu basis):
+ *   repeat: interface t^-1 = coherent mc; k-loop -> tau_c (BZ sum);
+ *           cpacor: per site (skipped at concentration 1 or mc = ma or mb)
+ *           xa = ((mc - ma)^-1 - tau_c)^-1, xb likewise,
+ *           mc -= ((c xa + (1-c) xb)^-1 + tau_c)^-1,
+ *           tau_a = tau_c + tau_c xa tau_c, tau_b likewise,
+ *           error = max over sites of |dmc|_F / |mc|_F;
+ *   until error <= cpatol or itcpam passes (cpacor's concpa / cpacon).
+ * All of it runs on the device.  When the device holds every k-point of the
+ * mesh (PL blocks, lead self-energies, Y_1: ~(32 + 2P) n^2 x 16 B per k), the
+ * passes after the first rerun only the PL sweeps, as the reference reuses
+ * its saved g2d and SSPOs (cpacoord.f:168-199); otherwise each pass is a full
+ * k-loop.  The leads' t^-1 are those of skkr_set_tinv; the interface t^-1 of
+ * the context is left at the final mc.
+ *   conc            sites = n_pl * sites_per_pl concentrations of species A
+ *   tinv_a, tinv_b  complex lk x lk x sites, species A and B (tminva, tminvb)
+ *   tinv_c          complex lk x lk x sites: in, the starting coherent t^-1
+ *                   unless init_ata (then tata, fullcpa.f:2-24); out, mc
+ *   tau_c           the last pass's BZ sum (overwritten, not accumulated)
+ *   tau_a, tau_b    species-projected tau of the last cpacor
+ *   itcpa           passes run; cpaerr the last error
+ * Returns a k-loop error as skkr_tau_bz does, SKKR_ERR_SINGULAR for a zero
+ * pivot in the solver; lk <= 128. */
+int skkr_cpa_bz(skkr_ctx *ctx, double e_re, double e_im, int nk, const double *k,
+                const double *w, const double *conc, const double *tinv_a,
+                const double *tinv_b, double *tinv_c, int init_ata, int itcpam,
+                double cpatol, double *tau_c, double *tau_a, double *tau_b,
+                int *itcpa, double *cpaerr, int *iters_k, int *status_k,
+                skkr_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,13 +149,16 @@ def fourier(R, GR, k):
 
 
 def tau_bz(system, E, k, w, nworkers=1, want_tau_k=False, itermax=None,
-           errmax=None):
+           errmax=None, tinv_int=None):
     """The whole k-loop of one energy.  Returns dict with tau_bz (sites,lk,lk),
     trk (nk, sites) traces of the unweighted per-k tau, iters (nk, 2) and
-    status (nk,), plus tau_k (nk, sites, lk, lk) when asked."""
+    status (nk,), plus tau_k (nk, sites, lk, lk) when asked.  tinv_int
+    replaces the interface t^-1 of the system (the CPA's coherent one)."""
     cfg = system.cfg
     lk, s, P = cfg.lk, cfg.s, cfg.P
     tint, tl, tr = system.tinv(E)
+    if tinv_int is not None:
+        tint = np.asarray(tinv_int, np.complex128)
     R = np.ascontiguousarray(system.R, np.float64)
     gi, go = _cm(system.G_intra), _cm(system.G_inter)
     ti, tlb, trb = _cm(tint), _cm(tl), _cm(tr)
@@ -422,3 +425,27 @@ def cpacor(conc, mc, ma, mb, tauc):
     if st:
         raise RuntimeError("orc_cpacor: singular matrix")
     return _uncm(c), _uncm(ta), _uncm(tb), err.value
+
+
+def cpa_bz(system, E, k, w, conc, tinv_a, tinv_b, tinv_c=None, itcpam=50, cpatol=1e-10, nworkers=1):
+    """CPA loop of one energy (nrsp-scf/cpacoord.f:107-345): interface t^-1 =
+    coherent mc, k-loop -> tau_c, cpacor; until error <= cpatol or itcpam
+    passes.  tinv_c None starts from tata.  Returns tau_c, tau_a, tau_b,
+    tinv_c (the updated mc), passes and error."""
+    if tinv_c is None:
+        mc = np.array([tata(c, a, b) for c, a, b in zip(conc, tinv_a, tinv_b)])
+    else:
+        mc = np.array(tinv_c, np.complex128)
+    itcpa = 1
+    while True:
+        r = tau_bz(system, E, k, w, nworkers=nworkers, tinv_int=mc)
+        if r["rc"]:
+            raise RuntimeError(f"oracle k-loop status {r['rc']}")
+        tauc = r["tau_bz"]
+        mc, ta, tb, err = cpacor(conc, mc, tinv_a, tinv_b, tauc)
+        concpa, cpacon = err > cpatol, itcpa < itcpam
+        itcpa += 1
+        if not (concpa and cpacon):
+            break
+    return {"tau_c": tauc, "tau_a": ta, "tau_b": tb, "tinv_c": mc, "itcpa": itcpa - 1, "error": err,
+            "iters": r["iters"]}

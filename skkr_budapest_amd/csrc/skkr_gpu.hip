@@ -25,6 +25,7 @@
 #include "gjinv_blk.h"
 #include "kkrlat.h"
 #include "kkrstr.h"
+#include "cpa.h"
 
 #define HIPCHK(x)                                                          \
   do {                                                                     \
@@ -126,6 +127,17 @@ struct skkr_ctx {
   PtrArr p_mt;                    // per chain: MtL (left), MtR (right)
   int mt_cap = 0;
   double e_re = 0.0, e_im = 0.0;  // energy of the current tau_bz call
+  // CPA loop: 1 = keep the batch's blocks and lead self-energies, 2 = reuse
+  // them (sweeps only; the k-points are those of the previous pass)
+  int cpa_mode = 0;
+  z* Y1s = nullptr;  // Y_1 of every k (the sweeps overwrite Ycur)
+  int y1_cap = 0;
+  int cpa_S = 0, cpa_lk = 0;  // sites and block size of the CPA buffers
+  double *dConc = nullptr, *dErr3 = nullptr;
+  int *dFlag = nullptr, *dInvSt = nullptr;
+  z *dMa = nullptr, *dMb = nullptr, *dDa = nullptr, *dDb = nullptr, *dXab = nullptr, *dH = nullptr,
+    *dT1 = nullptr, *dTa = nullptr, *dTb = nullptr;
+  PtrArr p_ma, p_mb, p_mc, p_da, p_db, p_xab, p_xa, p_xb, p_h, p_t1, p_tauc, p_ta, p_tb;
 };
 
 // ------------------------------------------------------------------------
@@ -834,23 +846,36 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B);
   std::vector<double> h_tr(2 * sites * B);
   std::vector<double> h_ph;
+  // the CPA passes keep every k resident in one batch
+  if (c->cpa_mode && nk > B) return SKKR_ERR_ARG;
+  const bool reuse = c->cpa_mode == 2;
   for (int k0 = 0; k0 < nk; k0 += B) {
     const int nb = std::min(B, nk - k0);
     ++nbatches;
-    HIPCHK(hipMemcpyAsync(c->dk, k + 2 * (size_t)k0, 2 * sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->dw, w + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
-    for (int j = 0; j < 2 * B; ++j) act[j] = (j % B) < nb ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
-    HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+    if (!reuse) {
+      HIPCHK(hipMemcpyAsync(c->dk, k + 2 * (size_t)k0, 2 * sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->dw, w + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+      for (int j = 0; j < 2 * B; ++j) act[j] = (j % B) < nb ? 1 : 0;
+      HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+      HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+    }
     HIPCHK(hipMemsetAsync(c->status_sw, 0, B * sizeof(int), c->stream));
     hipEventRecord(ev[0], c->stream);
-    if (c->str)
-      CHK(str_blocks(c, nb));
-    else
-      CHK(run_fourier(c, nb));
+    if (!reuse) {
+      if (c->str)
+        CHK(str_blocks(c, nb));
+      else
+        CHK(run_fourier(c, nb));
+    }
     hipEventRecord(ev[1], c->stream);
-    CHK(run_decimation(c, true, true, c->str ? &c->p_mt : nullptr));
+    if (!reuse) {
+      CHK(run_decimation(c, true, true, c->str ? &c->p_mt : nullptr));
+      if (c->cpa_mode == 1)
+        HIPCHK(hipMemcpyAsync(c->Y1s, c->Ycur, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(c->Ycur, c->Y1s, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
+    }
     hipEventRecord(ev[2], c->stream);
     CHK(run_sweeps(c, nb));
     hipEventRecord(ev[3], c->stream);
@@ -1626,6 +1651,219 @@ int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alph
       if (code != SKKR_OK && rc == SKKR_OK) rc = code;
     }
   }
+  return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// CPA loop of one energy (nrsp-scf/cpacoord.f:107-345 with fullcpa.f)
+// ------------------------------------------------------------------------
+// small inversions with full row pivoting (the CPA matrices have no
+// definite anti-Hermitian part): the register kernel of the row-wise path
+static int launch_gjinv_pivot(skkr_ctx* c, const InvArgs& a, int nb) {
+  if (nb <= 0) return SKKR_OK;
+  if (a.n <= 32)
+    gjinv_f64_kernel<8, 8, 4, 4><<<nb, 64, 0, c->stream>>>(a);
+  else if (a.n <= 64)
+    gjinv_f64_kernel<16, 16, 4, 4><<<nb, 256, 0, c->stream>>>(a);
+  else if (a.n <= 128)
+    gjinv_f64_kernel<16, 32, 8, 4><<<nb, 512, 0, c->stream>>>(a);
+  else
+    return SKKR_ERR_UNSUPPORTED;
+  HIPCHK(hipGetLastError());
+  c->launches++;
+  return SKKR_OK;
+}
+
+static int cpa_buffers(skkr_ctx* c) {
+  const int S = c->cfg.n_pl * c->ns, lk = c->cfg.lk;
+  if (c->cpa_S == S && c->cpa_lk == lk) return SKKR_OK;
+  const size_t nn = (size_t)lk * lk, tot = nn * S;
+  CHK(dalloc_t(c, &c->dConc, (size_t)S));
+  CHK(dalloc_t(c, &c->dErr3, (size_t)S));
+  CHK(dalloc_t(c, &c->dFlag, (size_t)2 * S));
+  CHK(dalloc_t(c, &c->dInvSt, (size_t)2 * S));
+  CHK(dalloc_t(c, &c->dMa, tot));
+  CHK(dalloc_t(c, &c->dMb, tot));
+  CHK(dalloc_t(c, &c->dDa, tot));
+  CHK(dalloc_t(c, &c->dDb, tot));
+  CHK(dalloc_t(c, &c->dXab, 2 * tot));
+  CHK(dalloc_t(c, &c->dH, tot));
+  CHK(dalloc_t(c, &c->dT1, 2 * tot));
+  CHK(dalloc_t(c, &c->dTa, tot));
+  CHK(dalloc_t(c, &c->dTb, tot));
+  auto arr = [&](PtrArr* pa, z* base, int cnt) {
+    std::vector<z*> v(cnt);
+    for (int j = 0; j < cnt; ++j) v[j] = base + (size_t)j * nn;
+    return make_ptrs(c, pa, v);
+  };
+  CHK(arr(&c->p_ma, c->dMa, S));
+  CHK(arr(&c->p_mb, c->dMb, S));
+  CHK(arr(&c->p_mc, c->dTint, S));
+  CHK(arr(&c->p_da, c->dDa, S));
+  CHK(arr(&c->p_db, c->dDb, S));
+  CHK(arr(&c->p_xab, c->dXab, 2 * S));
+  CHK(arr(&c->p_xa, c->dXab, S));
+  CHK(arr(&c->p_xb, c->dXab + tot, S));
+  CHK(arr(&c->p_h, c->dH, S));
+  CHK(arr(&c->p_t1, c->dT1, 2 * S));
+  CHK(arr(&c->p_tauc, c->taubz, S));
+  CHK(arr(&c->p_ta, c->dTa, S));
+  CHK(arr(&c->p_tb, c->dTb, S));
+  c->cpa_S = S;
+  c->cpa_lk = lk;
+  return SKKR_OK;
+}
+
+// tata (fullcpa.f:2-24) into the interface t^-1: mc = mb (c mb + (1-c) ma)^-1 ma
+static int cpa_ata(skkr_ctx* c) {
+  const int S = c->cpa_S, lk = c->cpa_lk;
+  const size_t nn = (size_t)lk * lk;
+  const long long tot = (long long)nn * S;
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  cpa::cpa_ata_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>((int)nn, tot, c->dConc, c->dMa, c->dMb,
+                                                                           c->dH);
+  HIPCHK(hipGetLastError());
+  InvArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = lk;
+  ia.lk = lk;
+  ia.base = c->p_h.d;
+  ia.out = c->p_h.d;
+  ia.status = c->dInvSt;  // |= 1 on a zero pivot
+  HIPCHK(hipMemsetAsync(c->dInvSt, 0, sizeof(int) * S, c->stream));
+  CHK(launch_gjinv_pivot(c, ia, S));
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_mb.d, lk, c->p_h.d, lk, c->p_t1.d, lk, one, zero, S, nullptr));
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_t1.d, lk, c->p_ma.d, lk, c->p_mc.d, lk, one, zero, S, nullptr));
+  std::vector<int> hs(S);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->dInvSt, sizeof(int) * S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int v : hs)
+    if (v & 1) return SKKR_ERR_SINGULAR;
+  return SKKR_OK;
+}
+
+// cpacor / cpasol (fullcpa.f:25-153) on the device; returns the largest
+// error3 of the updated sites in *err and SKKR_ERR_SINGULAR on a zero pivot
+static int cpa_step(skkr_ctx* c, double* err) {
+  const int S = c->cpa_S, lk = c->cpa_lk;
+  const size_t nn = (size_t)lk * lk;
+  const long long tot = (long long)nn * S;
+  const unsigned g = (unsigned)((tot + 255) / 256);
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  int* flag2 = c->dFlag;  // [flags of the sites | the same again for the xb half]
+  cpa::cpa_prep_kernel<<<S, 256, 0, c->stream>>>(lk, c->dConc, c->dMa, c->dMb, c->dTint, flag2, c->dDa, c->dDb,
+                                                c->dXab, c->dXab + tot);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(flag2 + S, flag2, sizeof(int) * S, hipMemcpyDeviceToDevice, c->stream));
+  int* st = c->dInvSt;
+  HIPCHK(hipMemsetAsync(st, 0, sizeof(int) * 2 * S, c->stream));
+  InvArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = lk;
+  ia.lk = lk;
+  ia.base = c->p_xab.d;
+  ia.out = c->p_xab.d;
+  ia.active = flag2;
+  ia.status = st;
+  // xa = ((mc - ma)^-1 - tauc)^-1, xb likewise (fullcpa.f:97-106)
+  CHK(launch_gjinv_pivot(c, ia, 2 * S));
+  cpa::cpa_sub_tau_kernel<<<g, 256, 0, c->stream>>>(tot, c->taubz, c->dXab, c->dXab + tot);
+  HIPCHK(hipGetLastError());
+  CHK(launch_gjinv_pivot(c, ia, 2 * S));
+  // h = ((c xa + (1-c) xb)^-1 + tauc)^-1, mc -= h (fullcpa.f:108-122)
+  cpa::cpa_mix_kernel<<<g, 256, 0, c->stream>>>((int)nn, tot, c->dConc, c->dXab, c->dXab + tot, c->dH);
+  HIPCHK(hipGetLastError());
+  ia.base = c->p_h.d;
+  ia.out = c->p_h.d;
+  CHK(launch_gjinv_pivot(c, ia, S));
+  cpa::cpa_add_tau_kernel<<<g, 256, 0, c->stream>>>(tot, c->taubz, c->dH);
+  HIPCHK(hipGetLastError());
+  CHK(launch_gjinv_pivot(c, ia, S));
+  cpa::cpa_update_kernel<<<S, 256, 0, c->stream>>>(lk, flag2, c->dH, c->taubz, c->dTint, c->dTa, c->dTb, c->dErr3);
+  HIPCHK(hipGetLastError());
+  // taua = tauc xa tauc + tauc, taub likewise (fullcpa.f:141-148)
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_tauc.d, lk, c->p_xa.d, lk, c->p_t1.d, lk, one, zero, S, flag2));
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_t1.d, lk, c->p_tauc.d, lk, c->p_ta.d, lk, one, one, S, flag2));
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_tauc.d, lk, c->p_xb.d, lk, c->p_t1.d, lk, one, zero, S, flag2));
+  CHK(launch_zgemm(c, lk, lk, lk, c->p_t1.d, lk, c->p_tauc.d, lk, c->p_tb.d, lk, one, one, S, flag2));
+  std::vector<double> e3(S);
+  std::vector<int> hs(2 * S);
+  HIPCHK(hipMemcpyAsync(e3.data(), c->dErr3, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hs.data(), st, sizeof(int) * 2 * S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double e = 0.0;
+  for (double v : e3) e = std::max(e, v);
+  *err = e;
+  for (int v : hs)
+    if (v & 1) return SKKR_ERR_SINGULAR;
+  return SKKR_OK;
+}
+
+extern "C" {
+
+int skkr_cpa_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k, const double* w,
+                const double* conc, const double* tinv_a, const double* tinv_b, double* tinv_c, int init_ata,
+                int itcpam, double cpatol, double* tau_c, double* tau_a, double* tau_b, int* itcpa_out,
+                double* cpaerr, int* iters_k, int* status_k, skkr_stats* st) {
+  if (!c || nk <= 0 || !k || !w || !conc || !tinv_a || !tinv_b || !tinv_c || !tau_c || !tau_a || !tau_b ||
+      itcpam < 1 || !(cpatol > 0.0))
+    return SKKR_ERR_ARG;
+  if (!(c->have_structure || c->str) || !c->have_tinv) return SKKR_ERR_ARG;
+  if (c->cfg.lk > 128) return SKKR_ERR_UNSUPPORTED;
+  const int S = c->cfg.n_pl * c->ns, lk = c->cfg.lk;
+  for (int i = 0; i < S; ++i)
+    if (!(conc[i] > 0.0 && conc[i] <= 1.0)) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  // every k resident when the device holds them (blocks, lead self-energies
+  // and Y_1 are kept across the CPA passes); else each pass recomputes them
+  const double per_k = (32.0 + 2.0 * c->cfg.n_pl) * c->nn * 16.0;
+  size_t freeb = 0, totb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totb));
+  const double have = (double)freeb + (c->B >= nk ? per_k * c->B : 0.0);
+  // (an explicit skkr_config.batch below nk asks for batches: no residency)
+  const bool resident = per_k * nk < 0.85 * have && (c->cfg.batch <= 0 || c->cfg.batch >= nk);
+  if (resident) {
+    CHK(ensure_batch(c, std::max(c->B, nk)));
+    if (c->y1_cap < c->B) {
+      CHK(dalloc_t(c, &c->Y1s, (size_t)c->B * c->nn));
+      c->y1_cap = c->B;
+    }
+  }
+  CHK(cpa_buffers(c));
+  const size_t nn = (size_t)lk * lk, tot = nn * S;
+  HIPCHK(hipMemcpyAsync(c->dConc, conc, sizeof(double) * S, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dMa, tinv_a, sizeof(z) * tot, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dMb, tinv_b, sizeof(z) * tot, hipMemcpyHostToDevice, c->stream));
+  if (init_ata)
+    CHK(cpa_ata(c));
+  else
+    HIPCHK(hipMemcpyAsync(c->dTint, tinv_c, sizeof(z) * tot, hipMemcpyHostToDevice, c->stream));
+  c->e_re = e_re;
+  c->e_im = e_im;
+  std::vector<double> hbz(2 * tot);
+  int itcpa = 1, rc = SKKR_OK;
+  double err = 0.0;
+  for (;;) {
+    c->cpa_mode = resident ? (itcpa == 1 ? 1 : 2) : 0;
+    std::fill(hbz.begin(), hbz.end(), 0.0);
+    rc = tau_bz_run(c, nk, k, w, hbz.data(), nullptr, iters_k, status_k, st, false, nullptr, nullptr);
+    if (rc != SKKR_OK) break;
+    rc = cpa_step(c, &err);
+    if (rc != SKKR_OK) break;
+    const bool concpa = err > cpatol, cpacon = itcpa < itcpam;
+    ++itcpa;
+    if (!(concpa && cpacon)) break;
+  }
+  c->cpa_mode = 0;
+  if (itcpa_out) *itcpa_out = itcpa - 1;
+  if (cpaerr) *cpaerr = err;
+  memcpy(tau_c, hbz.data(), sizeof(z) * tot);
+  HIPCHK(hipMemcpyAsync(tinv_c, c->dTint, sizeof(z) * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(tau_a, c->dTa, sizeof(z) * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(tau_b, c->dTb, sizeof(z) * tot, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return rc;
 }
 

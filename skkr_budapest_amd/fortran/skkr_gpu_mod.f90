@@ -132,5 +132,24 @@ module skkr_gpu
        type(c_ptr), value :: ctx, sc
        complex(c_double_complex), intent(in) :: alpha(*)
      end function skkr_set_structure_kkr
+
+     ! CPA loop of one energy (cpacoord.f:107-345, fullcpa.f)
+     integer(c_int) function skkr_cpa_bz(ctx, e_re, e_im, nk, k, w, conc, tinv_a, tinv_b, &
+          tinv_c, init_ata, itcpam, cpatol, tau_c, tau_a, tau_b, itcpa, cpaerr, iters_k, &
+          status_k, stats) bind(C, name="skkr_cpa_bz")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       real(c_double), value :: e_re, e_im
+       integer(c_int), value :: nk
+       real(c_double), intent(in) :: k(*), w(*), conc(*)
+       complex(c_double_complex), intent(in) :: tinv_a(*), tinv_b(*)
+       complex(c_double_complex), intent(inout) :: tinv_c(*)
+       integer(c_int), value :: init_ata, itcpam
+       real(c_double), value :: cpatol
+       complex(c_double_complex), intent(out) :: tau_c(*), tau_a(*), tau_b(*)
+       integer(c_int), intent(out) :: itcpa
+       real(c_double), intent(out) :: cpaerr
+       type(c_ptr), value :: iters_k, status_k, stats
+     end function skkr_cpa_bz
   end interface
 end module skkr_gpu

@@ -34,7 +34,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
-           "skkr_set_structure_kkr")
+           "skkr_set_structure_kkr", "skkr_cpa_bz")
 
 
 class SkkrError(RuntimeError):
@@ -110,6 +110,8 @@ def lib():
         L.skkr_str_destroy.restype = None
         L.skkr_gstore_batch.argtypes = [V, D, D, V, I, V, V, V]
         L.skkr_set_structure_kkr.argtypes = [V, V, V]
+        L.skkr_cpa_bz.argtypes = [V, D, D, I, V, V, V, V, V, V, I, I, D, V, V, V, V, V, V, V,
+                                  ctypes.POINTER(Stats)]
         _LIB = L
     return _LIB
 
@@ -186,6 +188,35 @@ class Context:
             raise ValueError("alpha must be s x (lmax+1)")
         self._check(lib().skkr_set_structure_kkr(self._h, structure._h, _p(al)), "skkr_set_structure_kkr")
         self._str = structure  # keeps the producer alive while bound
+
+    def cpa_bz(self, E, k, w, conc, tinv_a, tinv_b, tinv_c=None, itcpam=50, cpatol=1e-10,
+               raise_on_error=True):
+        """CPA loop of one energy (skkr_cpa_bz): conc[site], tinv_a/b
+        [site, lk, lk]; tinv_c None starts from the average t-matrix (tata).
+        Returns tau_c, tau_a, tau_b, the coherent tinv_c, passes and error."""
+        k = np.ascontiguousarray(k, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        nk = len(w)
+        ns = self.P * self.s
+        shp = (ns, self.lk, self.lk)
+        cc = np.ascontiguousarray(conc, np.float64)
+        a, b = colmajor(_c128(tinv_a)), colmajor(_c128(tinv_b))
+        mc = colmajor(_c128(tinv_c) if tinv_c is not None else np.zeros(shp, np.complex128))
+        tc, ta, tb = (colmajor(np.zeros(shp, np.complex128)) for _ in range(3))
+        itc = ctypes.c_int(0)
+        err = ctypes.c_double(0.0)
+        iters = np.zeros((nk, 2), np.int32)
+        status = np.zeros(nk, np.int32)
+        st = Stats()
+        rc = lib().skkr_cpa_bz(self._h, float(np.real(E)), float(np.imag(E)), nk, _p(k), _p(w), _p(cc),
+                               _p(a), _p(b), _p(mc), 1 if tinv_c is None else 0, int(itcpam),
+                               float(cpatol), _p(tc), _p(ta), _p(tb), ctypes.byref(itc),
+                               ctypes.byref(err), _p(iters), _p(status), ctypes.byref(st))
+        if raise_on_error:
+            self._check(rc, "skkr_cpa_bz")
+        return {"tau_c": from_colmajor(tc), "tau_a": from_colmajor(ta), "tau_b": from_colmajor(tb),
+                "tinv_c": from_colmajor(mc), "itcpa": itc.value, "error": err.value, "iters": iters,
+                "status": status, "rc": rc, "stats": st.as_dict()}
 
     def set_option(self, key, value):
         self._check(lib().skkr_set_option(self._h, key.encode(), int(value)),

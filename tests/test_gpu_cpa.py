@@ -1,0 +1,91 @@
+"""GPU CPA loop (skkr_cpa_bz through the C-ABI) against the oracle's CPA loop
+(oracle.cpa_bz: the oracle k-loop with the coherent interface t^-1 and
+orc_cpacor, itself pinned to the reference's fullcpa.f).  Tolerance 1e-10
+relative: the k-loop parity bound of the small systems, carried through a few
+CPA passes."""
+import numpy as np
+import pytest
+
+import oracle
+from skkr_budapest_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+def _alloy(lk=6, s=2, P=3, seed=5):
+    sy = S.make_system(S.small_config(lk=lk, s=s, P=P, r2max=8, tmodel="herm", energies=(0.4 + 0.05j,)))
+    E = sy.cfg.energies[0]
+    ti, _, _ = sy.tinv(E)
+    rng = np.random.default_rng(seed)
+    ns = ti.shape[0]
+    ta = ti + np.array([np.diag(rng.uniform(-0.3, 0.3, lk)) for _ in range(ns)])
+    tb = ti + np.array([np.diag(rng.uniform(-0.3, 0.3, lk)) for _ in range(ns)]) - 0.05j * np.eye(lk)
+    conc = rng.uniform(0.1, 0.9, ns)
+    conc[1] = 1.0  # a pure site: cpacor leaves it alone
+    return sy, E, conc, ta, tb
+
+
+@pytest.mark.parametrize("batch", [0, 8])
+def test_cpa_loop_matches_oracle(gpu, batch):
+    """batch 0: every k resident, later passes rerun only the sweeps;
+    batch 8: each pass a full k-loop in batches."""
+    sy, E, conc, ta, tb = _alloy()
+    k, w = S.uniform_kmesh(6)
+    ref = oracle.cpa_bz(sy, E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10, nworkers=4)
+    print(f"cpa loop lk=6 s=2 P=3 nk={len(w)} batch={batch}", flush=True)
+    with gpu.Context.for_system(sy, batch=batch) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.cpa_bz(E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10)
+    assert r["itcpa"] == ref["itcpa"]
+    assert r["error"] <= 1e-10
+    for key in ("tau_c", "tau_a", "tau_b", "tinv_c"):
+        assert rel(r[key], ref[key]) < TOL, key
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    # the CPA condition holds at the end
+    c = conc[:, None, None]
+    assert rel(c * r["tau_a"] + (1 - c) * r["tau_b"], r["tau_c"]) < 1e-9
+
+
+def test_cpa_pure_sites_stop_after_one_pass(gpu):
+    sy, E, conc, ta, tb = _alloy(seed=6)
+    conc[:] = 1.0
+    k, w = S.uniform_kmesh(4)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.cpa_bz(E, k, w, conc, ta, tb, itcpam=10, cpatol=1e-10)
+        ref = ctx.tau_bz(E, k, w)
+    assert r["itcpa"] == 1 and r["error"] == 0.0
+    np.testing.assert_array_equal(r["tau_a"], r["tau_c"])
+    np.testing.assert_array_equal(r["tau_b"], r["tau_c"])
+    assert rel(r["tinv_c"], ta) < 1e-14          # tata of a pure site is t_A^-1
+    # tau_c of the loop equals a plain k-loop with interface t^-1 = t_A^-1
+    ti, tl, tr = sy.tinv(E)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(ta, tl, tr)
+        plain = ctx.tau_bz(E, k, w)
+    assert rel(r["tau_c"], plain["tau_bz"]) < 1e-13
+
+
+def test_cpa_larger_system_matches_oracle(gpu):
+    """lk 16, 2 sites per PL, 4 PLs (n = 32), 8 x 8 k."""
+    sy, E, conc, ta, tb = _alloy(lk=16, s=2, P=4, seed=7)
+    k, w = S.uniform_kmesh(8)
+    ref = oracle.cpa_bz(sy, E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10, nworkers=8)
+    print("cpa loop lk=16 s=2 P=4 nk=64", flush=True)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.cpa_bz(E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10)
+    assert r["itcpa"] == ref["itcpa"]
+    for key in ("tau_c", "tau_a", "tau_b", "tinv_c"):
+        assert rel(r[key], ref[key]) < TOL, key
