@@ -449,3 +449,58 @@ def cpa_bz(system, E, k, w, conc, tinv_a, tinv_b, tinv_c=None, itcpam=50, cpatol
             break
     return {"tau_c": tauc, "tau_a": ta, "tau_b": tb, "tinv_c": mc, "itcpa": itcpa - 1, "error": err,
             "iters": r["iters"]}
+
+
+def full_tau_k(system, E, k, itermax=None, errmax=None):
+    """tau between every pair of interface sites at one k, [li, lj, lk, lk]:
+    the dense inverse (LAPACK through numpy) of the finite stack of P
+    principal layers with the lead self-energies of dugo on the outer PLs --
+    the matrix whose blocks tau2dgod (skkr_dbogo_bsf/clu/tau2d_god.f) builds
+    by its recursion."""
+    cfg = system.cfg
+    lk, s, P = cfg.lk, cfg.s, cfg.P
+    n = lk * s
+    tint, tl, tr = system.tinv(E)
+    gi = fourier(system.R, system.G_intra, k)
+    go = fourier(system.R, system.G_inter, k)
+    m00, m01, m10 = pl_blocks(gi, go)
+    it, em = int(itermax or cfg.itermax), float(errmax or cfg.errmax)
+    y, _, st1 = dugo(m00, m01, m10, tl, "L", it, em)
+    x, _, st2 = dugo(m00, m01, m10, tr, "R", it, em)
+    if st1 or st2:
+        raise RuntimeError("decimation failed")
+    A = np.zeros((P * n, P * n), np.complex128)
+    for p in range(P):
+        sl = slice(p * n, (p + 1) * n)
+        A[sl, sl] = m00
+        for i in range(s):
+            si = slice(p * n + i * lk, p * n + (i + 1) * lk)
+            A[si, si] += tint[p * s + i]
+        if p + 1 < P:
+            A[sl, (p + 1) * n:(p + 2) * n] = m01
+            A[(p + 1) * n:(p + 2) * n, sl] = m10
+    A[:n, :n] -= y
+    A[(P - 1) * n:, (P - 1) * n:] -= x
+    T = np.linalg.inv(A)
+    ns = P * s
+    return T.reshape(ns, lk, ns, lk).transpose(0, 2, 1, 3).copy()
+
+
+def tau_pairs(system, E, ks, ws, pairs, a1, a2):
+    """BZ sums of the off-diagonal tau of interface-site pairs as the cluster
+    code accumulates them (skkr_dbogo_bsf/clu/cpacoord_save.f:432-478 with the
+    identity point operation): pair (li, lj, n1, n2), 0-based sites,
+    R = n1 a1 + n2 a2:  tau_ij = sum_k w e^{+i k.R} tau_{li,lj}(k),
+    tau_ji = sum_k w e^{-i k.R} tau_{lj,li}(k)."""
+    lk = system.cfg.lk
+    tij = np.zeros((len(pairs), lk, lk), np.complex128)
+    tji = np.zeros_like(tij)
+    a1, a2 = np.asarray(a1, float), np.asarray(a2, float)
+    for k, w in zip(ks, ws):
+        T = full_tau_k(system, E, k)
+        for q, (li, lj, n1, n2) in enumerate(pairs):
+            R = n1 * a1 + n2 * a2
+            ph = np.exp(1j * (k @ R))
+            tij[q] += w * ph * T[li, lj]
+            tji[q] += w * np.conj(ph) * T[lj, li]
+    return tij, tji

@@ -15,6 +15,8 @@ c  tinvl(lk,lk,s), tinvr(lk,lk,s); real*8 w(nk); then per k
 c  complex*16 gintra(n,n), ginter(n,n) with n=lk*s (G, not -G).
 c  Output: complex*16 taubz(lk,lk,P*s), trk(P*s,nk), tausave(lk,lk,P*s,nsave).
 c  Decimation iterations are printed by surfgfd (ichk=1) on stdout.
+c  -DGOD (with -DDBOGO): tau2dgod of skkr_dbogo_bsf/clu/tau2d_god.f instead
+c  of tau2d, and tausave holds the full tau(lk,lk,P*s,P*s) of the saved k.
       program refdrv
       implicit real*8 (a-h,o-z)
       include 'param.h'
@@ -28,6 +30,9 @@ c  Decimation iterations are printed by surfgfd (ichk=1) on stdout.
       complex*16, allocatable :: tinv(:,:,:),tinvl(:,:,:),tinvr(:,:,:)
       complex*16, allocatable :: tau1(:,:,:),taubz(:,:,:)
       complex*16, allocatable :: trk(:,:),tsave(:,:,:,:)
+#ifdef GOD
+      complex*16, allocatable :: tauf(:,:,:,:),tfsave(:,:,:,:,:)
+#endif
       complex*16, allocatable :: gi(:,:),go(:,:)
       real*8, allocatable :: w(:)
       integer, allocatable :: isave(:)
@@ -61,6 +66,10 @@ c
       allocate(tau1(lk,lk,mintfc),taubz(lk,lk,np*ns))
       allocate(trk(np*ns,nk),tsave(lk,lk,np*ns,max(nsave,1)))
       allocate(gi(n,n),go(n,n),w(nk))
+#ifdef GOD
+      allocate(tauf(lk,lk,np*ns,np*ns))
+      allocate(tfsave(lk,lk,np*ns,np*ns,max(nsave,1)))
+#endif
       tinv=(0.d0,0.d0)
       tinvl=(0.d0,0.d0)
       tinvr=(0.d0,0.d0)
@@ -126,8 +135,19 @@ c
         call dugo(lmax,lk,tinvr,lk,g2d(1,1,-minprc,1,2),
      &            g2d(1,1,-minprc,1,1),xx,mdim,'R',.true.,0,sgfconv)
         if(.not.sgfconv) stop 3
+#ifdef GOD
+        call tau2dgod(0,1.d0,.true.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
+     &                tauf,mdim)
+        do li=1,np*ns
+          tau1(:,:,li)=tauf(:,:,li,li)
+        end do
+        do is=1,nsave
+          if(isave(is).eq.ik) tfsave(:,:,:,:,is)=tauf
+        end do
+#else
         call tau2d(0,1.d0,.true.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
      &             tau1,mdim)
+#endif
         do li=1,np*ns
           trk(li,ik)=(0.d0,0.d0)
           do l=1,lk
@@ -149,6 +169,10 @@ c
      &     status='replace')
       write(12) taubz
       write(12) trk
+#ifdef GOD
+      if(nsave.gt.0) write(12) tfsave(:,:,:,:,1:nsave)
+#else
       if(nsave.gt.0) write(12) tsave(:,:,:,1:nsave)
+#endif
       close(12)
       end
