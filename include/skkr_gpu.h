@@ -299,6 +299,26 @@ int skkr_cpa_bz(skkr_ctx *ctx, double e_re, double e_im, int nk, const double *k
                 int *itcpa, double *cpaerr, int *iters_k, int *status_k,
                 skkr_stats *stats);
 
+/* ------------------------------------------------------------------------
+ * Off-diagonal tau of interface-site pairs, summed over the BZ with the
+ * phase of a lateral lattice vector: the input of the cluster embedding,
+ * skkr_dbogo_bsf/clu/cpacoord_save.f:432-478 (identity point operation) with
+ * the full tau of clu/tau2d_god.f:2-270:
+ *   tau_ij(pair) += sum_k w_k e^{+i k.R} tau_{li,lj}(k),
+ *   tau_ji(pair) += sum_k w_k e^{-i k.R} tau_{lj,li}(k),  R = n1 a1 + n2 a2.
+ * pairs: 4 x npair ints (li, lj: 1-based interface sites as kpair(3:4),
+ * n1, n2 as kpair(1:2)); a1, a2: the 2D Bravais vectors (common/brav2d/).
+ * Blocks between principal layers p < q come from the sweeps' Z_t = g_t M01
+ * and G_q, p > q from W_t = M10 g_t and G_p (the recursion of tau2dgod).
+ * skkr_tau_bz_pairs is skkr_tau_bz plus these sums (tau_ij, tau_ji complex
+ * lk x lk x npair, accumulated). */
+int skkr_set_pairs(skkr_ctx *ctx, int npair, const int *pairs, const double *a1,
+                   const double *a2);
+int skkr_tau_bz_pairs(skkr_ctx *ctx, double e_re, double e_im, int nk,
+                      const double *k, const double *w, double *tau_bz,
+                      double *tau_ij, double *tau_ji, int *iters_k,
+                      int *status_k, skkr_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

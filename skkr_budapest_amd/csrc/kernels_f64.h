@@ -614,6 +614,28 @@ __global__ __launch_bounds__(256) void bz_accumulate_kernel(
   acc[e] = s;
 }
 
+// Off-diagonal BZ sums of site pairs: acc[e] += sum_b w[b] e^{sgn i k_b.R}
+// T_b[i lk + a, j lk + c], e = a + lk c, b in order (deterministic); T_b are
+// the batch's n x n blocks of tau between the two principal layers.
+__global__ __launch_bounds__(256) void pair_accumulate_kernel(
+    int lk, int n, int nb, const double2* const* __restrict__ T, int i, int j, double rx, double ry,
+    double sgn, const double* __restrict__ kpts, const double* __restrict__ w, double2* __restrict__ acc) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= lk * lk) return;
+  const int a = e % lk, c = e / lk;
+  const long long off = (long long)(i * lk + a) + (long long)(j * lk + c) * n;
+  double2 s = acc[e];
+  for (int b = 0; b < nb; ++b) {
+    double sn, cs;
+    sincos(kpts[2 * b] * rx + kpts[2 * b + 1] * ry, &sn, &cs);
+    const double2 t = T[b][off];
+    const double pr = w[b] * cs, pi = w[b] * sgn * sn;
+    s.x += pr * t.x - pi * t.y;
+    s.y += pr * t.y + pi * t.x;
+  }
+  acc[e] = s;
+}
+
 __global__ __launch_bounds__(256) void scale_copy_kernel(
     long long ne, const double2* __restrict__ src, double scale, double2* __restrict__ dst) {
   long long e = blockIdx.x * 256ll + threadIdx.x;

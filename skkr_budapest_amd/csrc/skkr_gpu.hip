@@ -138,6 +138,15 @@ struct skkr_ctx {
   z *dMa = nullptr, *dMb = nullptr, *dDa = nullptr, *dDb = nullptr, *dXab = nullptr, *dH = nullptr,
     *dT1 = nullptr, *dTa = nullptr, *dTb = nullptr;
   PtrArr p_ma, p_mb, p_mc, p_da, p_db, p_xab, p_xa, p_xb, p_h, p_t1, p_tauc, p_ta, p_tb;
+  // site pairs of the off-diagonal BZ sums: (li, lj) 0-based, R = n1 a1 + n2 a2
+  std::vector<int> pair_li, pair_lj;
+  std::vector<double> pair_rx, pair_ry;
+  bool pairs_on = false;  // the sweeps keep W_p = M10 g_p of every PL
+  z* Ws = nullptr;
+  int ws_cap = 0;
+  std::vector<PtrArr> p_Ws;
+  z *dTij = nullptr, *dTji = nullptr;  // npair x lk x lk accumulators
+  int pair_cap = 0;
 };
 
 // ------------------------------------------------------------------------
@@ -620,9 +629,10 @@ static int sweep_launches(skkr_ctx* c, int nb) {
     CHK(launch_extract(c, P - 1, nb));
   }
   for (int p = P - 2; p >= 0; --p) {
-    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_G[p].d, n, c->p_W.d, n, one, zero, nb,
+    z** W = c->pairs_on ? c->p_Ws[p].d : c->p_W.d;
+    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_G[p].d, n, W, n, one, zero, nb,
                      nullptr));
-    CHK(launch_zgemm(c, n, n, n, c->p_G[p + 1].d, n, c->p_W.d, n, c->p_Z.d, n, one, zero, nb,
+    CHK(launch_zgemm(c, n, n, n, c->p_G[p + 1].d, n, W, n, c->p_Z.d, n, one, zero, nb,
                      nullptr));
     CHK(launch_zgemm(c, n, n, n, c->p_Zs[p].d, n, c->p_Z.d, n, c->p_G[p].d, n, one, one, nb,
                      nullptr));
@@ -634,7 +644,7 @@ static int sweep_launches(skkr_ctx* c, int nb) {
 // The sweeps of a batch, optionally as a captured HIP graph (one per nb; all
 // buffers and pointer arrays are fixed for the context's batch size).
 static int run_sweeps(skkr_ctx* c, int nb) {
-  if (!c->graphs) return sweep_launches(c, nb);
+  if (!c->graphs || c->pairs_on) return sweep_launches(c, nb);
   auto it = c->sweep_graphs.find(nb);
   if (it == c->sweep_graphs.end()) {
     hipGraph_t g = nullptr;
@@ -799,6 +809,77 @@ static int auto_batch(skkr_ctx* c, int nk) {
 // tau_bz_phys += phystau of this call's BZ sum, taukdiag = per-k per-l traces.
 static int str_blocks(skkr_ctx* c, int nb);
 
+// W_p of every PL for the pair sums
+static int ensure_ws(skkr_ctx* c) {
+  const int P = c->cfg.n_pl;
+  if (c->ws_cap >= c->B || P < 2) return SKKR_OK;
+  const size_t nn = c->nn, Bs = c->B;
+  CHK(dalloc_t(c, &c->Ws, (size_t)(P - 1) * Bs * nn));
+  c->p_Ws.assign(P - 1, PtrArr());
+  for (int p = 0; p < P - 1; ++p) {
+    std::vector<z*> v(Bs);
+    for (size_t j = 0; j < Bs; ++j) v[j] = c->Ws + ((size_t)p * Bs + j) * nn;
+    CHK(make_ptrs(c, &c->p_Ws[p], v));
+  }
+  c->ws_cap = c->B;
+  return SKKR_OK;
+}
+
+// Off-diagonal BZ sums of the site pairs for the nb k-points of a batch
+// (after the sweeps).  Blocks of tau between PLs p and q of every k:
+//   p < q: tau_pq = (-Z_p)(-Z_{p+1}) ... (-Z_{q-1}) G_q,
+//   p > q: tau_pq = G_p (-W_{p-1}) (-W_{p-2}) ... (-W_q),
+// with Z_t = g_t M01 and W_t = M10 g_t of the sweeps -- the same blocks as
+// tau2dgod's recursion (clu/tau2d_god.f) -- built right-to-left (p < q) or
+// left-to-right (p > q) in the free W / Z scratch; then each pair's lk x lk
+// block is summed over k with its phase.
+static int pair_sums(skkr_ctx* c, int nb) {
+  const int n = c->n, lk = c->cfg.lk, s = c->ns, np = (int)c->pair_li.size();
+  const z mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  std::vector<std::pair<int, int>> done;
+  for (int q2 = 0; q2 < np; ++q2)
+    for (int dir = 0; dir < 2; ++dir) {
+      const int la = dir ? c->pair_lj[q2] : c->pair_li[q2], lb = dir ? c->pair_li[q2] : c->pair_lj[q2];
+      const int p = la / s, q = lb / s;
+      std::pair<int, int> key(p, q);
+      bool seen = false;
+      for (auto& d : done) seen = seen || (d == key);
+      if (seen) continue;
+      done.push_back(key);
+      z** T = c->p_G[p].d;
+      if (p < q) {
+        z** buf[2] = {c->p_W.d, c->p_Z.d};
+        z** cur = c->p_G[q].d;
+        for (int t = q - 1, h = 0; t >= p; --t, h ^= 1) {
+          CHK(launch_zgemm(c, n, n, n, c->p_Zs[t].d, n, cur, n, buf[h], n, mone, zero, nb, nullptr));
+          cur = buf[h];
+        }
+        T = cur;
+      } else if (p > q) {
+        z** buf[2] = {c->p_W.d, c->p_Z.d};
+        z** cur = c->p_G[p].d;
+        for (int t = p - 1, h = 0; t >= q; --t, h ^= 1) {
+          CHK(launch_zgemm(c, n, n, n, cur, n, c->p_Ws[t].d, n, buf[h], n, mone, zero, nb, nullptr));
+          cur = buf[h];
+        }
+        T = cur;
+      }
+      // every pair (either direction) whose blocks lie between p and q
+      for (int r = 0; r < np; ++r)
+        for (int d2 = 0; d2 < 2; ++d2) {
+          const int xa = d2 ? c->pair_lj[r] : c->pair_li[r], xb = d2 ? c->pair_li[r] : c->pair_lj[r];
+          if (xa / s != p || xb / s != q) continue;
+          z* acc = (d2 ? c->dTji : c->dTij) + (size_t)r * lk * lk;
+          pair_accumulate_kernel<<<(lk * lk + 255) / 256, 256, 0, c->stream>>>(
+              lk, n, nb, (const double2* const*)T, xa % s, xb % s, c->pair_rx[r], c->pair_ry[r], d2 ? -1.0 : 1.0,
+              c->dk, c->dw, acc);
+          HIPCHK(hipGetLastError());
+          c->launches++;
+        }
+    }
+  return SKKR_OK;
+}
+
 // outer self-energy factors of the chains (layer producer only)
 static int ensure_mt(skkr_ctx* c) {
   if (c->mt_cap >= c->B) return SKKR_OK;
@@ -822,6 +903,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   auto t0 = std::chrono::steady_clock::now();
   CHK(ensure_batch(c, auto_batch(c, std::max(nk, 1))));
   if (c->str) CHK(ensure_mt(c));
+  if (c->pairs_on) CHK(ensure_ws(c));
   // the chain layout on the device (left chains 0..B-1, right chains B..2B-1)
   // follows the allocated batch, which an earlier call may have made larger
   const int B = c->B;
@@ -878,6 +960,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     }
     hipEventRecord(ev[2], c->stream);
     CHK(run_sweeps(c, nb));
+    if (c->pairs_on) CHK(pair_sums(c, nb));
     hipEventRecord(ev[3], c->stream);
     {
       int ne = (int)tsz;
@@ -1864,6 +1947,58 @@ int skkr_cpa_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k, 
   HIPCHK(hipMemcpyAsync(tau_a, c->dTa, sizeof(z) * tot, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(tau_b, c->dTb, sizeof(z) * tot, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+
+int skkr_set_pairs(skkr_ctx* c, int npair, const int* pairs, const double* a1, const double* a2) {
+  if (!c || npair < 0 || (npair > 0 && (!pairs || !a1 || !a2))) return SKKR_ERR_ARG;
+  const int S = c->cfg.n_pl * c->ns, lk = c->cfg.lk;
+  for (int q = 0; q < npair; ++q)
+    if (pairs[4 * q] < 1 || pairs[4 * q] > S || pairs[4 * q + 1] < 1 || pairs[4 * q + 1] > S) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  c->pair_li.resize(npair);
+  c->pair_lj.resize(npair);
+  c->pair_rx.resize(npair);
+  c->pair_ry.resize(npair);
+  for (int q = 0; q < npair; ++q) {
+    c->pair_li[q] = pairs[4 * q] - 1;
+    c->pair_lj[q] = pairs[4 * q + 1] - 1;
+    const double n1 = pairs[4 * q + 2], n2 = pairs[4 * q + 3];
+    c->pair_rx[q] = n1 * a1[0] + n2 * a2[0];
+    c->pair_ry[q] = n1 * a1[1] + n2 * a2[1];
+  }
+  if (npair > c->pair_cap) {
+    CHK(dalloc_t(c, &c->dTij, (size_t)npair * lk * lk));
+    CHK(dalloc_t(c, &c->dTji, (size_t)npair * lk * lk));
+    c->pair_cap = npair;
+  }
+  return SKKR_OK;
+}
+
+int skkr_tau_bz_pairs(skkr_ctx* c, double e_re, double e_im, int nk, const double* k, const double* w,
+                      double* tau_bz, double* tau_ij, double* tau_ji, int* iters_k, int* status_k,
+                      skkr_stats* st) {
+  if (!c || !tau_ij || !tau_ji) return SKKR_ERR_ARG;
+  const int np = (int)c->pair_li.size(), lk = c->cfg.lk;
+  if (np == 0) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const size_t ne = (size_t)np * lk * lk;
+  HIPCHK(hipMemsetAsync(c->dTij, 0, sizeof(z) * ne, c->stream));
+  HIPCHK(hipMemsetAsync(c->dTji, 0, sizeof(z) * ne, c->stream));
+  c->e_re = e_re;
+  c->e_im = e_im;
+  c->pairs_on = true;
+  const int rc = tau_bz_run(c, nk, k, w, tau_bz, nullptr, iters_k, status_k, st, false, nullptr, nullptr);
+  c->pairs_on = false;
+  std::vector<double> hij(2 * ne), hji(2 * ne);
+  HIPCHK(hipMemcpyAsync(hij.data(), c->dTij, sizeof(z) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hji.data(), c->dTji, sizeof(z) * ne, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (size_t e = 0; e < 2 * ne; ++e) {
+    tau_ij[e] += hij[e];
+    tau_ji[e] += hji[e];
+  }
   return rc;
 }
 

@@ -151,5 +151,25 @@ module skkr_gpu
        real(c_double), intent(out) :: cpaerr
        type(c_ptr), value :: iters_k, status_k, stats
      end function skkr_cpa_bz
+
+     ! off-diagonal tau of interface-site pairs (clu/cpacoord_save.f:432-478)
+     integer(c_int) function skkr_set_pairs(ctx, npair, pairs, a1, a2) bind(C, name="skkr_set_pairs")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: npair
+       integer(c_int), intent(in) :: pairs(*)
+       real(c_double), intent(in) :: a1(2), a2(2)
+     end function skkr_set_pairs
+
+     integer(c_int) function skkr_tau_bz_pairs(ctx, e_re, e_im, nk, k, w, tau_bz, tau_ij, tau_ji, &
+          iters_k, status_k, stats) bind(C, name="skkr_tau_bz_pairs")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       real(c_double), value :: e_re, e_im
+       integer(c_int), value :: nk
+       real(c_double), intent(in) :: k(*), w(*)
+       complex(c_double_complex), intent(inout) :: tau_bz(*), tau_ij(*), tau_ji(*)
+       type(c_ptr), value :: iters_k, status_k, stats
+     end function skkr_tau_bz_pairs
   end interface
 end module skkr_gpu

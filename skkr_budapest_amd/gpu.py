@@ -34,7 +34,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
-           "skkr_set_structure_kkr", "skkr_cpa_bz")
+           "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs")
 
 
 class SkkrError(RuntimeError):
@@ -112,6 +112,8 @@ def lib():
         L.skkr_set_structure_kkr.argtypes = [V, V, V]
         L.skkr_cpa_bz.argtypes = [V, D, D, I, V, V, V, V, V, V, I, I, D, V, V, V, V, V, V, V,
                                   ctypes.POINTER(Stats)]
+        L.skkr_set_pairs.argtypes = [V, I, V, V, V]
+        L.skkr_tau_bz_pairs.argtypes = [V, D, D, I, V, V, V, V, V, V, V, ctypes.POINTER(Stats)]
         _LIB = L
     return _LIB
 
@@ -217,6 +219,31 @@ class Context:
         return {"tau_c": from_colmajor(tc), "tau_a": from_colmajor(ta), "tau_b": from_colmajor(tb),
                 "tinv_c": from_colmajor(mc), "itcpa": itc.value, "error": err.value, "iters": iters,
                 "status": status, "rc": rc, "stats": st.as_dict()}
+
+    def set_pairs(self, pairs, a1, a2):
+        """Site pairs (li, lj, n1, n2), 0-based sites, of the off-diagonal sums."""
+        pr = np.array([(li + 1, lj + 1, n1, n2) for li, lj, n1, n2 in pairs], np.int32).reshape(-1)
+        A1, A2 = np.ascontiguousarray(a1, np.float64), np.ascontiguousarray(a2, np.float64)
+        self._check(lib().skkr_set_pairs(self._h, len(pairs), _p(pr), _p(A1), _p(A2)), "skkr_set_pairs")
+        self._npair = len(pairs)
+
+    def tau_bz_pairs(self, E, k, w):
+        """skkr_tau_bz plus the pair sums: returns tau_bz, tau_ij, tau_ji."""
+        k = np.ascontiguousarray(k, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        nk = len(w)
+        ns = self.P * self.s
+        acc = colmajor(np.zeros((ns, self.lk, self.lk), np.complex128))
+        tij = colmajor(np.zeros((self._npair, self.lk, self.lk), np.complex128))
+        tji = colmajor(np.zeros((self._npair, self.lk, self.lk), np.complex128))
+        iters = np.zeros((nk, 2), np.int32)
+        status = np.zeros(nk, np.int32)
+        st = Stats()
+        rc = lib().skkr_tau_bz_pairs(self._h, float(np.real(E)), float(np.imag(E)), nk, _p(k), _p(w),
+                                     _p(acc), _p(tij), _p(tji), _p(iters), _p(status), ctypes.byref(st))
+        self._check(rc, "skkr_tau_bz_pairs")
+        return {"tau_bz": from_colmajor(acc), "tau_ij": from_colmajor(tij), "tau_ji": from_colmajor(tji),
+                "iters": iters, "status": status, "stats": st.as_dict()}
 
     def set_option(self, key, value):
         self._check(lib().skkr_set_option(self._h, key.encode(), int(value)),
