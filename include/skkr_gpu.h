@@ -266,6 +266,15 @@ int skkr_gstore_batch(skkr_str *sc, double e_re, double e_im, const double *alph
  * sites_per_pl = s.  A producer serves one context at a time. */
 int skkr_set_structure_kkr(skkr_ctx *ctx, skkr_str *sc, const double *alpha);
 
+/* The same for the relativistic k-loop (rsp-scf): rel = 1 evaluates the
+ * structure constants at psq = E + E^2/c^2 (c = 274.072) and rel = 2 at E
+ * (rsp-scf/cpacoord_ksave.f:93-99, wrel), then transforms every lm block to
+ * the kappa-mu basis with the factor psq/E (packer.f with irel = 1,
+ * relmtrx.f:2-45, Clebsch-Gordan tables of clebsch.f:2-70); requires
+ * lk = 2 (lmax+1)^2.  rel = 0 is skkr_set_structure_kkr. */
+int skkr_set_structure_kkr_rel(skkr_ctx *ctx, skkr_str *sc, const double *alpha,
+                               int rel);
+
 /* ------------------------------------------------------------------------
  * CPA loop of one energy: nrsp-scf/cpacoord.f:107-345 around the k-loop,
  * with the solver of nrsp-scf/fullcpa.f (rsp-scf/fullcpa.f is the same in

@@ -323,6 +323,20 @@ class Lattice:
         return out.reshape(3, s, 2 * s + 1, nl2, nl2)
 
 
+def kkr_blocks_rel(g2d, s, lmax, fac):
+    """kkr_blocks in the kappa-mu basis: every lm block through relmtrx
+    (packer.f with irel = 1)."""
+    nl2 = (lmax + 1) ** 2
+    lk = 2 * nl2
+    g = np.zeros((3, s, 2 * s + 1, lk, lk), np.complex128)
+    for p in range(3):
+        for i in range(s):
+            for jj in range(2 * s + 1):
+                # relmtrx of the [m, n] block, stored back as [n, m] like g2d
+                g[p, i, jj] = relmtrx(g2d[p, i, jj].T, fac, lmax).T
+    return kkr_blocks(g, s, lk)
+
+
 def kkr_blocks(g2d, s, nl2):
     """packer.f:36-97 (equal ninprc, irel = 0) on the g2d planes of one k
     (array [icpl, icount, jj+s, n, m] as Lattice.gstore returns it): M00, M01,
@@ -351,23 +365,29 @@ def kkr_blocks(g2d, s, nl2):
 
 
 def kkr_tau_bz(lattice, lmax, s, nextra, cvec, alpha, E, eta, tinv, tinv_l, tinv_r, ks, ws,
-               itermax=50, errmax=1e-8):
+               itermax=50, errmax=1e-8, rel=0):
     """k-loop of nrsp-scf/cpacoord.f:147-311 with bulkgeo on the oracle:
     gstore (side 'L') -> packer -> dugo('L'), dugo('R') with the outer factors
     of planes 0 and 2 -> tau2d; BZ sum of the weighted site-diagonal tau.
     tinv: (P*s, nl2, nl2); tinv_l, tinv_r: (s, nl2, nl2).  Returns
     (tau_bz [sites, nl2, nl2], per-k decimation iterations [k, 2])."""
     nl2 = (lmax + 1) ** 2
+    lk = 2 * nl2 if rel else nl2
     P = tinv.shape[0] // s
-    acc = np.zeros((P * s, nl2, nl2), np.complex128)
+    acc = np.zeros((P * s, lk, lk), np.complex128)
     its = []
+    E = complex(E)
+    psq = E + E * E / 274.072 ** 2 if rel == 1 else E
     for k, w in zip(ks, ws):
-        g2d = lattice.gstore(lmax, s, nextra, cvec, alpha, E, k, eta)
-        m00, m01, m10, mtl, mtr = kkr_blocks(g2d, s, nl2)
+        g2d = lattice.gstore(lmax, s, nextra, cvec, alpha, psq, k, eta)
+        if rel:
+            m00, m01, m10, mtl, mtr = kkr_blocks_rel(g2d, s, lmax, psq / E)
+        else:
+            m00, m01, m10, mtl, mtr = kkr_blocks(g2d, s, nl2)
         tl = m00.copy()
         tr = m00.copy()
         for i in range(s):
-            sl = slice(i * nl2, (i + 1) * nl2)
+            sl = slice(i * lk, (i + 1) * lk)
             tl[sl, sl] += tinv_l[i]
             tr[sl, sl] += tinv_r[i]
         gl, itl, st1 = surfgfd(m10, tl, m01, itermax, errmax)
@@ -504,3 +524,46 @@ def tau_pairs(system, E, ks, ws, pairs, a1, a2):
             tij[q] += w * ph * T[li, lj]
             tji[q] += w * np.conj(ph) * T[lj, li]
     return tij, tji
+
+
+def clebsch(lmax=5):
+    """Clebsch-Gordan rectangular matrices of clebsch.f (lm -> kappa-mu):
+    u1, u2 and 0-based index maps ind1, ind2 over kappa-mu indices
+    ir = 2 kap^2 + kap + m (+1), l <= 5 (clebsch.f:21-67)."""
+    u1, u2 = np.zeros(72), np.zeros(72)
+    i1, i2 = np.zeros(72, np.int64), np.zeros(72, np.int64)
+    inr = 0
+    for l in range(6):
+        t = float(2 * l + 1)
+        for m in range(-l, l + 1):
+            inr += 1
+            if l != 0:
+                kap = l
+                ir = 2 * kap * kap + kap + m
+                u1[ir - 1], i1[ir - 1] = np.sqrt((l + m) / t), inr
+                ir += 1
+                u2[ir - 1], i2[ir - 1] = -np.sqrt((l - m) / t), inr
+            kap = -l - 1
+            ir = 2 * kap * kap + kap + m
+            u1[ir - 1], i1[ir - 1] = np.sqrt((l - m + 1) / t), inr
+            ir += 1
+            u2[ir - 1], i2[ir - 1] = np.sqrt((l + m + 1) / t), inr
+    i1[i1 == 0] = 1
+    i2[i2 == 0] = 1
+    return u1, u2, i1 - 1, i2 - 1
+
+
+def relmtrx(a, fac, lmax):
+    """relmtrx.f:2-45: b(i, j) = u1(i) (fac a)(ind1 i, ind1 j) u1(j)
+    + u2(i) (fac a)(ind2 i, ind2 j) u2(j), i, j over 2 (lmax+1)^2."""
+    u1, u2, i1, i2 = clebsch()
+    n, nl2 = 2 * (lmax + 1) ** 2, (lmax + 1) ** 2
+    # zero-padded like relmtrx.f's a1(36,36): entries with u = 0 may point
+    # past the lm block (clebsch.f reuses an index of the next l)
+    a1 = np.zeros((36, 36), np.complex128)
+    a1[:nl2, :nl2] = fac * np.asarray(a, np.complex128)
+    b = np.zeros((n, n), np.complex128)
+    for i in range(n):
+        for j in range(n):
+            b[i, j] = u1[i] * a1[i1[i], i1[j]] * u1[j] + u2[i] * a1[i2[i], i2[j]] * u2[j]
+    return b

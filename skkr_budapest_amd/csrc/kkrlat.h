@@ -279,4 +279,35 @@ static void ylm_tables(int l2max, std::vector<double>& b, std::vector<std::compl
     }
 }
 
+// lm -> kappa-mu rectangular Clebsch-Gordan matrices for l <= 5: u1, u2
+// and the lm index (0-based) of each kappa-mu index ir = 2 kap^2 + kap + m
+// (ms = -1/2) and ir + 1 (ms = +1/2), j = l -+ 1/2 (clebsch.f:21-67); an
+// unset index points at lm 0 with coefficient 0
+static void clebsch_tables(std::vector<double>& u1, std::vector<double>& u2, std::vector<int>& i1,
+                           std::vector<int>& i2) {
+  u1.assign(72, 0.0);
+  u2.assign(72, 0.0);
+  i1.assign(72, 0);
+  i2.assign(72, 0);
+  int inr = -1;
+  for (int l = 0; l <= 5; ++l) {
+    const double t = (double)(2 * l + 1);
+    for (int m = -l; m <= l; ++m) {
+      ++inr;
+      if (l != 0) {
+        const int kap = l, ir = 2 * kap * kap + kap + m;
+        u1[ir - 1] = std::sqrt((l + m) / t);
+        i1[ir - 1] = inr;
+        u2[ir] = -std::sqrt((l - m) / t);
+        i2[ir] = inr;
+      }
+      const int kap = -l - 1, ir = 2 * kap * kap + kap + m;
+      u1[ir - 1] = std::sqrt((l - m + 1) / t);
+      i1[ir - 1] = inr;
+      u2[ir] = std::sqrt((l + m + 1) / t);
+      i2[ir] = inr;
+    }
+  }
+}
+
 }  // namespace kkrh

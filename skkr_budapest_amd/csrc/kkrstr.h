@@ -796,6 +796,49 @@ __global__ __launch_bounds__(256) void kkr_pack_kernel(int N, int nl2, int s, in
   MtR[o] = (jp >= i) ? at(p2 + i, p1 + jp) : zero;
 }
 
+// The same blocks in the kappa-mu basis (lk = 2 nl2), packer.f with irel = 1:
+// each nl2 x nl2 block a goes through relmtrx.f:26-42,
+//   b(i, j) = u1(i) (f a)(ind1 i, ind1 j) u1(j) + u2(i) (f a)(ind2 i, ind2 j) u2(j),
+// f = psq / E; indices past the lm block carry coefficient 0.
+__global__ __launch_bounds__(256) void kkr_pack_rel_kernel(int N, int nl2, int s, int nextra,
+                                                           const double2* __restrict__ gs, int nb, long long nn,
+                                                           double fr, double fi, const double* __restrict__ u1,
+                                                           const double* __restrict__ u2, const int* __restrict__ i1,
+                                                           const int* __restrict__ i2, double2* M00, double2* M01,
+                                                           double2* M10, double2* MtL, double2* MtR) {
+  const int lk = 2 * nl2, n = lk * s;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int kb = blockIdx.y;
+  if (idx >= (long long)n * n || kb >= nb) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  const int i = r / lk, kr = r % lk, jp = c / lk, kc = c % lk;
+  const double2* g = gs + (size_t)kb * N * N;
+  const cx f = cx{fr, fi};
+  const double a1 = u1[kr], b1 = u1[kc], a2 = u2[kr], b2 = u2[kc];
+  const int p1 = i1[kr], q1 = i1[kc], p2 = i2[kr], q2 = i2[kc];
+  auto at = [&](int la, int lb) {
+    cx v1 = mk(0.0), v2 = mk(0.0);
+    if (p1 < nl2 && q1 < nl2) {
+      const double2 t = g[(size_t)(la * nl2 + p1) + (size_t)(lb * nl2 + q1) * N];
+      v1 = f * cx{-t.x, -t.y};
+    }
+    if (p2 < nl2 && q2 < nl2) {
+      const double2 t = g[(size_t)(la * nl2 + p2) + (size_t)(lb * nl2 + q2) * N];
+      v2 = f * cx{-t.x, -t.y};
+    }
+    const cx b = (a1 * v1) * b1 + (a2 * v2) * b2;
+    return make_double2(b.r, b.i);
+  };
+  const double2 zero = make_double2(0.0, 0.0);
+  const int p0 = nextra * s, pl1 = p0 + s, pl2 = pl1 + s;
+  const size_t o = (size_t)kb * nn + idx;
+  M00[o] = at(pl1 + i, pl1 + jp);
+  M01[o] = (jp <= i) ? at(pl1 + i, pl2 + jp) : zero;
+  M10[o] = (jp >= i) ? at(pl1 + i, p0 + jp) : zero;
+  MtL[o] = (jp <= i) ? at(p0 + i, pl1 + jp) : zero;
+  MtR[o] = (jp >= i) ? at(pl2 + i, pl1 + jp) : zero;
+}
+
 // producer status of a k -> k-loop status bits (8: not converged, 1: pivot)
 __global__ void kkr_status_kernel(int nb, const int* __restrict__ st, int* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;

@@ -125,6 +125,9 @@ struct skkr_ctx {
   z* strAlpha = nullptr;          // (lmax+1) x s screening parameters
   z *MtL = nullptr, *MtR = nullptr;  // outer factors of the lead self-energies
   PtrArr p_mt;                    // per chain: MtL (left), MtR (right)
+  int str_rel = 0;                // 0 lm basis; 1, 2 kappa-mu (psq = E + E^2/c^2, or E)
+  double *dU1 = nullptr, *dU2 = nullptr;
+  int *dI1 = nullptr, *dI2 = nullptr;
   int mt_cap = 0;
   double e_re = 0.0, e_im = 0.0;  // energy of the current tau_bz call
   // CPA loop: 1 = keep the batch's blocks and lead self-energies, 2 = reuse
@@ -1669,12 +1672,22 @@ static int str_blocks(skkr_ctx* c, int nb) {
   skkr_str* sc = c->str;
   for (int k0 = 0; k0 < nb; k0 += sc->B) {
     const int nbb = std::min(sc->B, nb - k0);
-    CHK(str_run(sc, c->stream, c->e_re, c->e_im, c->strAlpha, c->dk + 2 * (size_t)k0, nbb));
+    // the relativistic k-loop evaluates the structure constants at
+    // psq = E + E^2/c^2 and scales them by psq / E (rsp-scf/cpacoord_ksave.f:93-99)
+    std::complex<double> E(c->e_re, c->e_im), psq = E;
+    if (c->str_rel == 1) psq = E + E * E / (274.072 * 274.072);
+    const std::complex<double> fac = psq / E;
+    CHK(str_run(sc, c->stream, psq.real(), psq.imag(), c->strAlpha, c->dk + 2 * (size_t)k0, nbb));
     const size_t nn = c->nn, o = (size_t)k0 * nn;
     dim3 grid((unsigned)((nn + 255) / 256), nbb);
-    kkr::kkr_pack_kernel<<<grid, 256, 0, c->stream>>>(sc->N, sc->nl2, sc->cfg.s, sc->cfg.nextra, sc->d_gs, nbb,
-                                                       (long long)nn, c->M00 + o, c->M01 + o, c->M10 + o,
-                                                       c->MtL + o, c->MtR + o);
+    if (c->str_rel)
+      kkr::kkr_pack_rel_kernel<<<grid, 256, 0, c->stream>>>(
+          sc->N, sc->nl2, sc->cfg.s, sc->cfg.nextra, sc->d_gs, nbb, (long long)nn, fac.real(), fac.imag(), c->dU1,
+          c->dU2, c->dI1, c->dI2, c->M00 + o, c->M01 + o, c->M10 + o, c->MtL + o, c->MtR + o);
+    else
+      kkr::kkr_pack_kernel<<<grid, 256, 0, c->stream>>>(sc->N, sc->nl2, sc->cfg.s, sc->cfg.nextra, sc->d_gs, nbb,
+                                                         (long long)nn, c->M00 + o, c->M01 + o, c->M10 + o,
+                                                         c->MtL + o, c->MtR + o);
     HIPCHK(hipGetLastError());
     kkr::kkr_status_kernel<<<(nbb + 255) / 256, 256, 0, c->stream>>>(nbb, sc->d_status, c->status_sw + k0);
     HIPCHK(hipGetLastError());
@@ -1685,13 +1698,41 @@ static int str_blocks(skkr_ctx* c, int nb) {
 
 extern "C" {
 
+int skkr_set_structure_kkr_rel(skkr_ctx* c, skkr_str* sc, const double* alpha, int rel) {
+  if (!c || rel < 0 || rel > 2) return SKKR_ERR_ARG;
+  if (!sc) {
+    c->str = nullptr;
+    return SKKR_OK;
+  }
+  const int lk_need = rel ? 2 * sc->nl2 : sc->nl2;
+  if (!alpha || c->cfg.lk != lk_need || c->cfg.sites_per_pl != sc->cfg.s || c->cfg.device != sc->cfg.device)
+    return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  if (rel && !c->dU1) {
+    std::vector<double> u1, u2;
+    std::vector<int> i1, i2;
+    kkrh::clebsch_tables(u1, u2, i1, i2);
+    CHK(dalloc_t(c, &c->dU1, u1.size()));
+    CHK(dalloc_t(c, &c->dU2, u2.size()));
+    CHK(dalloc_t(c, &c->dI1, i1.size()));
+    CHK(dalloc_t(c, &c->dI2, i2.size()));
+    HIPCHK(hipMemcpy(c->dU1, u1.data(), sizeof(double) * u1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dU2, u2.data(), sizeof(double) * u2.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dI1, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dI2, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice));
+  }
+  c->str_rel = rel;
+  return skkr_set_structure_kkr(c, sc, alpha);
+}
+
 int skkr_set_structure_kkr(skkr_ctx* c, skkr_str* sc, const double* alpha) {
   if (!c) return SKKR_ERR_ARG;
   if (!sc) {
     c->str = nullptr;
     return SKKR_OK;
   }
-  if (!alpha || c->cfg.lk != sc->nl2 || c->cfg.sites_per_pl != sc->cfg.s || c->cfg.device != sc->cfg.device)
+  const int lk_need = c->str_rel ? 2 * sc->nl2 : sc->nl2;
+  if (!alpha || c->cfg.lk != lk_need || c->cfg.sites_per_pl != sc->cfg.s || c->cfg.device != sc->cfg.device)
     return SKKR_ERR_ARG;
   HIPCHK(hipSetDevice(c->cfg.device));
   const size_t na = (size_t)(sc->cfg.lmax + 1) * sc->cfg.s;

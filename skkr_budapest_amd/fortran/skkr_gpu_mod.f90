@@ -133,6 +133,15 @@ module skkr_gpu
        complex(c_double_complex), intent(in) :: alpha(*)
      end function skkr_set_structure_kkr
 
+     ! relativistic k-loop on the producer (rel 1: psq = E + E^2/c^2, 2: E)
+     integer(c_int) function skkr_set_structure_kkr_rel(ctx, sc, alpha, rel) &
+          bind(C, name="skkr_set_structure_kkr_rel")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx, sc
+       complex(c_double_complex), intent(in) :: alpha(*)
+       integer(c_int), value :: rel
+     end function skkr_set_structure_kkr_rel
+
      ! CPA loop of one energy (cpacoord.f:107-345, fullcpa.f)
      integer(c_int) function skkr_cpa_bz(ctx, e_re, e_im, nk, k, w, conc, tinv_a, tinv_b, &
           tinv_c, init_ata, itcpam, cpatol, tau_c, tau_a, tau_b, itcpa, cpaerr, iters_k, &

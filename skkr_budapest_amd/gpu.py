@@ -34,7 +34,8 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_zgemm_batch", "skkr_surfgfd_batch", "skkr_dugo_batch",
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
-           "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs")
+           "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs",
+           "skkr_set_structure_kkr_rel")
 
 
 class SkkrError(RuntimeError):
@@ -110,6 +111,7 @@ def lib():
         L.skkr_str_destroy.restype = None
         L.skkr_gstore_batch.argtypes = [V, D, D, V, I, V, V, V]
         L.skkr_set_structure_kkr.argtypes = [V, V, V]
+        L.skkr_set_structure_kkr_rel.argtypes = [V, V, V, I]
         L.skkr_cpa_bz.argtypes = [V, D, D, I, V, V, V, V, V, V, I, I, D, V, V, V, V, V, V, V,
                                   ctypes.POINTER(Stats)]
         L.skkr_set_pairs.argtypes = [V, I, V, V, V]
@@ -177,10 +179,12 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
-    def set_structure_kkr(self, structure, alpha):
+    def set_structure_kkr(self, structure, alpha, rel=0):
         """Take the structure constants of the k-loop from a ``Structure``
         (the layer producer) with screening alpha[layer, l] at the energy of
-        the next tau_bz calls; ``structure=None`` returns to the Fourier sum."""
+        the next tau_bz calls; ``structure=None`` returns to the Fourier sum.
+        rel 1 / 2: kappa-mu basis (lk = 2 (lmax+1)^2), constants at
+        E + E^2/c^2 / at E."""
         if structure is None:
             self._check(lib().skkr_set_structure_kkr(self._h, None, None), "skkr_set_structure_kkr")
             self._str = None
@@ -188,7 +192,8 @@ class Context:
         al = _c128(alpha)
         if al.shape != (structure.s, structure.lmax + 1):
             raise ValueError("alpha must be s x (lmax+1)")
-        self._check(lib().skkr_set_structure_kkr(self._h, structure._h, _p(al)), "skkr_set_structure_kkr")
+        self._check(lib().skkr_set_structure_kkr_rel(self._h, structure._h, _p(al), int(rel)),
+                    "skkr_set_structure_kkr_rel")
         self._str = structure  # keeps the producer alive while bound
 
     def cpa_bz(self, E, k, w, conc, tinv_a, tinv_b, tinv_c=None, itcpam=50, cpatol=1e-10,

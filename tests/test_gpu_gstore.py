@@ -144,3 +144,30 @@ def test_kloop_with_layer_structure_constants_matches_oracle(gpu):
             ctx.set_structure_kkr(None, None)
             with pytest.raises(G.SkkrError):  # no Fourier structure set either
                 ctx.tau_bz(E, ks, ws)
+
+
+@pytest.mark.parametrize("rel", [1, 2])
+def test_relativistic_kloop_with_layer_structure_constants(gpu, rel):
+    """The kappa-mu k-loop on the producer (skkr_set_structure_kkr_rel):
+    constants at E + E^2/c^2 (rel 1) or E (rel 2), blocks through relmtrx,
+    against the oracle's composition with relmtrx (pinned to clebsch.f /
+    relmtrx.f by tests/golden/relmtrx.npz)."""
+    G = gpu
+    E = complex(0.5, 0.05)
+    g, al, ti = _kkr_system(E)
+    s, P, nl2 = g["s"], 2, (LMAX + 1) ** 2
+    tir = np.array([oracle.relmtrx(t, 1.0, LMAX) for t in ti])
+    tinv = np.concatenate([tir] * P)
+    ks = np.array([[0.2, -0.1], [-0.45, 0.3], [0.05, 0.6]])
+    ws = np.full(len(ks), 1.0 / len(ks))
+    L = oracle.Lattice(g["a1"], g["a2"])
+    ref, its = oracle.kkr_tau_bz(L, LMAX, s, g["nextra"], g["cvec"], al, E, g["eta"], tinv, tir, tir, ks, ws,
+                                 rel=rel)
+    print(f"relativistic k-loop on the producer, rel={rel}", flush=True)
+    with G.Structure(LMAX, s, g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        with G.Context(lk=2 * nl2, sites_per_pl=s, n_pl=P) as ctx:
+            ctx.set_tinv(tinv, tir, tir)
+            ctx.set_structure_kkr(st, al, rel=rel)
+            r = ctx.tau_bz(E, ks, ws)
+    np.testing.assert_array_equal(r["iters"], its)
+    assert np.abs(r["tau_bz"] - ref).max() < TOL * np.abs(ref).max()

@@ -101,3 +101,24 @@ def test_screened_g2d_matches_reference(geo):
             al = GOLD[p + ("alpha_l" if side == "L" else "alpha_r")]
             g = L.gstore(LMAX, s, nextra, GOLD[p + "cvec"], al, E, k, eta)
             assert rel(g, GOLD[p + f"g2d_{side}"][ik]) < 1e-11
+
+
+RELG = np.load(os.path.join(os.path.dirname(__file__), "golden", "relmtrx.npz"), allow_pickle=False)
+
+
+def test_relmtrx_matches_reference():
+    for a, fac, b in zip(RELG["a"], RELG["fac"], RELG["b"]):
+        assert np.abs(oracle.relmtrx(a, fac, int(RELG["lmax"])) - b).max() < 1e-14 * np.abs(b).max()
+
+
+def test_relmtrx_is_a_unitary_change_of_basis():
+    """The Clebsch-Gordan matrices are orthogonal: relmtrx(I) = f I, and the
+    spectrum of a becomes that of a (x) I_2 scaled by f."""
+    for lmax in (1, 2, 3):
+        n = (lmax + 1) ** 2
+        assert np.abs(oracle.relmtrx(np.eye(n), 2.5, lmax) - 2.5 * np.eye(2 * n)).max() < 1e-14
+    rng = np.random.default_rng(8)
+    a = rng.normal(size=(9, 9)) + 1j * rng.normal(size=(9, 9))
+    ev = np.sort_complex(np.linalg.eigvals(oracle.relmtrx(a, 1.0, 2)))
+    ref = np.sort_complex(np.concatenate([np.linalg.eigvals(a)] * 2))
+    assert np.abs(ev - ref).max() < 1e-12 * np.abs(ref).max()
