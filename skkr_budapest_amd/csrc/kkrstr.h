@@ -839,6 +839,16 @@ __global__ __launch_bounds__(256) void kkr_pack_rel_kernel(int N, int nl2, int s
   MtR[o] = (jp >= i) ? at(pl2 + i, pl1 + jp) : zero;
 }
 
+// dst[b] (ldd) = src[b] (lds), an m x n block per batch entry
+__global__ __launch_bounds__(256) void kkr_copy_block_kernel(int m, int n, const double2* const* __restrict__ src,
+                                                             int lds, double2* const* __restrict__ dst, int ldd) {
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int b = blockIdx.y;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  dst[b][r + (long long)c * ldd] = src[b][r + (long long)c * lds];
+}
+
 // producer status of a k -> k-loop status bits (8: not converged, 1: pivot)
 __global__ void kkr_status_kernel(int nb, const int* __restrict__ st, int* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;

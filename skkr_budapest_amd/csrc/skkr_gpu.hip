@@ -1383,7 +1383,11 @@ struct skkr_str {
   int nl2 = 0, nlay = 0, N = 0, B = 0, nkam = 0, ndir = 0, nblk = 0;
   double rlmin = 0.0;
   kkrh::Lattice lat;
-  skkr_ctx* inv = nullptr;  // batched inversion and products of size N
+  skkr_ctx* inv = nullptr;   // batched inversion and products of size min(N, 256)
+  skkr_ctx* invS = nullptr;  // N > 384: the (N - 256)-row Schur complement
+  // N > 256: 2 x 2 blocks around the leading 256 rows (views into d_g1)
+  z *d_X11 = nullptr, *d_C = nullptr, *d_D = nullptr, *d_Si = nullptr;
+  PtrArr p_X11, p_C, p_D, p_Si, p_A11, p_A21, p_A12, p_A22;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   // tables
@@ -1439,7 +1443,7 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
     return SKKR_ERR_ARG;
   if (cfg->lmax > 3) return SKKR_ERR_UNSUPPORTED;
   const int nl2 = (cfg->lmax + 1) * (cfg->lmax + 1), nlay = (2 * cfg->nextra + 3) * cfg->s;
-  if (nl2 * nlay > 256) return SKKR_ERR_UNSUPPORTED;
+  if (nl2 * nlay > 512) return SKKR_ERR_UNSUPPORTED;
   const double cell = cfg->a1[0] * cfg->a2[1] - cfg->a1[1] * cfg->a2[0];
   if (!(std::fabs(cell) > 0.0)) return SKKR_ERR_ARG;
   int ndev = 0;
@@ -1458,7 +1462,7 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
   // inversion / product engine of size N
   skkr_config ic;
   memset(&ic, 0, sizeof(ic));
-  ic.lk = sc->N;
+  ic.lk = std::min(sc->N, 256);
   ic.sites_per_pl = 1;
   ic.n_pl = 1;
   ic.errmax = 1e-8;
@@ -1467,6 +1471,11 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
   ic.device = cfg->device;
   int r = skkr_create(&sc->inv, &ic);
   if (r != SKKR_OK) return fail(r);
+  if (sc->N - 256 > 128) {
+    ic.lk = sc->N - 256;
+    r = skkr_create(&sc->invS, &ic);
+    if (r != SKKR_OK) return fail(r);
+  }
   sc->stream = sc->inv->stream;
   // lattice and the stack of layers (gstore.f:99-122)
   sc->lat = kkrh::make_lattice(cfg->a1, cfg->a2, 20000);
@@ -1577,6 +1586,18 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
       (r = str_ptrs(sc, &sc->p_g1, sc->d_g1, NN, B)) || (r = str_ptrs(sc, &sc->p_gs, sc->d_gs, NN, B)) ||
       (r = ensure_inverse(sc->inv, B)))
     return fail(r);
+  if (N > 256) {
+    const size_t h1 = 256, h2 = N - 256;
+    if ((r = str_alloc(sc, &sc->d_X11, B * h1 * h1)) || (r = str_alloc(sc, &sc->d_C, B * h2 * h1)) ||
+        (r = str_alloc(sc, &sc->d_D, B * h1 * h2)) || (r = str_alloc(sc, &sc->d_Si, B * h2 * h2)) ||
+        (r = str_ptrs(sc, &sc->p_X11, sc->d_X11, h1 * h1, B)) || (r = str_ptrs(sc, &sc->p_C, sc->d_C, h2 * h1, B)) ||
+        (r = str_ptrs(sc, &sc->p_D, sc->d_D, h1 * h2, B)) || (r = str_ptrs(sc, &sc->p_Si, sc->d_Si, h2 * h2, B)) ||
+        (r = str_ptrs(sc, &sc->p_A11, sc->d_g1, NN, B)) || (r = str_ptrs(sc, &sc->p_A21, sc->d_g1 + h1, NN, B)) ||
+        (r = str_ptrs(sc, &sc->p_A12, sc->d_g1 + h1 * N, NN, B)) ||
+        (r = str_ptrs(sc, &sc->p_A22, sc->d_g1 + h1 + h1 * N, NN, B)) ||
+        (sc->invS && (r = ensure_inverse(sc->invS, B))))
+      return fail(r);
+  }
   *out = sc;
   return SKKR_OK;
 }
@@ -1587,10 +1608,64 @@ void skkr_str_destroy(skkr_str* sc) {
   if (sc->stream) hipStreamSynchronize(sc->stream);
   for (void* p : sc->allocs) hipFree(p);
   if (sc->inv) skkr_destroy(sc->inv);
+  if (sc->invS) skkr_destroy(sc->invS);
   delete sc;
 }
 
 }  // extern "C"
+
+// In-place inverse of 1 - alpha G for 256 < N <= 512 (on sc->inv's stream):
+// 2 x 2 blocks around A11 = the leading 256 rows,
+//   X11 = A11^-1, C = A21 X11, S = A22 - C A12, D = X11 A12,
+//   B12 = -D S^-1, B21 = -S^-1 C, B11 = X11 - B12 C, B22 = S^-1,
+// each block written over the one it replaces once that one is consumed.
+// No rows move between the halves: the matrices are those the screening
+// transformation inverts, pivoted within the halves.
+static int str_inverse_large(skkr_str* sc, int nb) {
+  const int N = sc->N, h1 = 256, h2 = N - 256;
+  skkr_ctx* e = sc->inv;
+  const z one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  InvArgs ia;
+  memset(&ia, 0, sizeof(ia));
+  ia.n = h1;
+  ia.lk = h1;
+  ia.ld = N;
+  ia.base = sc->p_A11.d;
+  ia.out = sc->p_X11.d;
+  ia.status = sc->d_status;
+  CHK(run_inverse(e, ia, nb));
+  CHK(launch_zgemm(e, h2, h1, h1, sc->p_A21.d, N, sc->p_X11.d, h1, sc->p_C.d, h2, one, zero, nb, nullptr));
+  CHK(launch_zgemm(e, h2, h2, h1, sc->p_C.d, h2, sc->p_A12.d, N, sc->p_A22.d, N, mone, one, nb, nullptr));
+  InvArgs is;
+  memset(&is, 0, sizeof(is));
+  is.n = h2;
+  is.lk = h2;
+  is.ld = N;
+  is.base = sc->p_A22.d;
+  is.out = sc->p_Si.d;
+  is.status = sc->d_status;
+  if (h2 <= 128) {
+    is.ldo = h2;
+    CHK(launch_gjinv(e, is, nb));
+  } else {
+    CHK(run_inverse(sc->invS, is, nb));
+  }
+  CHK(launch_zgemm(e, h1, h2, h1, sc->p_X11.d, h1, sc->p_A12.d, N, sc->p_D.d, h1, one, zero, nb, nullptr));
+  CHK(launch_zgemm(e, h1, h2, h2, sc->p_D.d, h1, sc->p_Si.d, h2, sc->p_A12.d, N, mone, zero, nb, nullptr));
+  CHK(launch_zgemm(e, h2, h1, h2, sc->p_Si.d, h2, sc->p_C.d, h2, sc->p_A21.d, N, mone, zero, nb, nullptr));
+  {
+    dim3 grid((unsigned)(((long long)h1 * h1 + 255) / 256), nb);
+    kkr::kkr_copy_block_kernel<<<grid, 256, 0, e->stream>>>(h1, h1, (const double2* const*)sc->p_X11.d, h1,
+                                                            sc->p_A11.d, N);
+    HIPCHK(hipGetLastError());
+    dim3 grid2((unsigned)(((long long)h2 * h2 + 255) / 256), nb);
+    kkr::kkr_copy_block_kernel<<<grid2, 256, 0, e->stream>>>(h2, h2, (const double2* const*)sc->p_Si.d, h2,
+                                                             sc->p_A22.d, N);
+    HIPCHK(hipGetLastError());
+  }
+  CHK(launch_zgemm(e, h1, h1, h2, sc->p_A12.d, N, sc->p_C.d, h2, sc->p_A11.d, N, mone, one, nb, nullptr));
+  return SKKR_OK;
+}
 
 // screened constants of nb <= sc->B k-points (device k) into sc->d_gs, on
 // stream; status per k in sc->d_status
@@ -1648,21 +1723,26 @@ static int str_run(skkr_str* sc, hipStream_t stream, double e_re, double e_im, c
                                                        sc->nblk, alpha, sc->d_lof, sc->d_g, sc->d_g1, nb);
     HIPCHK(hipGetLastError());
   }
-  // the inversion and the product run on the engine's launch paths, on this stream
-  hipStream_t own = sc->inv->stream;
+  // the inversion and the product run on the engines' launch paths, on this stream
+  hipStream_t own = sc->inv->stream, ownS = sc->invS ? sc->invS->stream : nullptr;
   sc->inv->stream = stream;
-  InvArgs ia;
-  memset(&ia, 0, sizeof(ia));
-  ia.n = N;
-  ia.lk = N;
-  ia.base = sc->p_g1.d;
-  ia.out = sc->p_g1.d;
-  ia.status = sc->d_status;
-  int r = run_inverse(sc->inv, ia, nb);
+  if (sc->invS) sc->invS->stream = stream;
+  int r = (N <= 256) ? [&] {
+    InvArgs ia;
+    memset(&ia, 0, sizeof(ia));
+    ia.n = N;
+    ia.lk = N;
+    ia.base = sc->p_g1.d;
+    ia.out = sc->p_g1.d;
+    ia.status = sc->d_status;
+    return run_inverse(sc->inv, ia, nb);
+  }()
+                     : str_inverse_large(sc, nb);
   if (r == SKKR_OK)
     r = launch_zgemm(sc->inv, N, N, N, sc->p_g.d, N, sc->p_g1.d, N, sc->p_gs.d, N, make_double2(1, 0),
                      make_double2(0, 0), nb, nullptr);
   sc->inv->stream = own;
+  if (sc->invS) sc->invS->stream = ownS;
   return r;
 }
 

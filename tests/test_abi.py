@@ -99,7 +99,7 @@ def test_structure_constant_entry_points_validate_arguments_without_device():
     for (lmax, s, nextra, eta, code) in ((2, 2, 0, 1.0, 1),   # nextra < 1
                                          (2, 2, 1, 0.0, 1),   # eta <= 0
                                          (4, 2, 1, 1.0, 7),   # lmax > 3
-                                         (3, 4, 1, 1.0, 7)):  # nl2 (2 nextra + 3) s > 256
+                                         (3, 4, 3, 1.0, 7)):  # nl2 (2 nextra + 3) s > 512
         cfg = gpu.StrConfig(lmax, s, nextra, eta, A(5.0, 0.0), A(0.0, 5.0), 0, 0)
         assert lib.skkr_str_create(ctypes.byref(h), ctypes.byref(cfg), cv) == code
     lib.skkr_gstore_batch.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,

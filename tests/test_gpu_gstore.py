@@ -171,3 +171,21 @@ def test_relativistic_kloop_with_layer_structure_constants(gpu, rel):
             r = ctx.tau_bz(E, ks, ws)
     np.testing.assert_array_equal(r["iters"], its)
     assert np.abs(r["tau_bz"] - ref).max() < TOL * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("nextra", [1, 2])
+def test_gstore_beyond_256_rows(gpu, nextra):
+    """lmax 3 on the Nb(110) stack: 1 - alpha G has 320 (nextra 1) or 448
+    (nextra 2) rows, inverted by 2 x 2 blocks around the leading 256."""
+    from skkr_budapest_amd import kkrsystems as K
+    G = gpu
+    g = _geo("nb110")
+    al = np.array([K.square_well_t(g["E"], K.NB110["rmt"], 3.0, 3) for _ in range(g["s"])])
+    ks = np.array([[0.21, -0.4]])
+    L = oracle.Lattice(g["a1"], g["a2"])
+    print(f"gstore nb110 lmax=3 nextra={nextra}", flush=True)
+    with G.Structure(3, g["s"], nextra, g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        out, stat = st.gstore(g["E"], al, ks)
+    assert (stat == 0).all()
+    ref = L.gstore(3, g["s"], nextra, g["cvec"], al, g["E"], ks[0], g["eta"])
+    assert rel(out[0], ref) < TOL
