@@ -167,15 +167,16 @@ def kkr_bench(args):
     from skkr_budapest_amd import gpu
     from skkr_budapest_amd import kkrsystems as K
     E = complex(0.5, 0.05)
-    wl = K.nb110_workload(E)
+    rel = 1 if args.kkr_rel else 0
+    wl = K.nb110_workload(E, P=20, lmax=3, rel=1) if rel else K.nb110_workload(E)
     st = wl["structure"]
     k, w = K.reciprocal_mesh(st["a1"], st["a2"], 64)
     nk = len(w)
-    print(f"bench: kkr Nb(110) nk={nk} E={E}", file=sys.stderr, flush=True)
-    with gpu.Structure(**st) as sc, gpu.Context(lk=(st["lmax"] + 1) ** 2, sites_per_pl=st["s"],
-                                                 n_pl=wl["P"]) as ctx:
+    lk = (2 if rel else 1) * (st["lmax"] + 1) ** 2
+    print(f"bench: kkr Nb(110) lmax={st['lmax']} rel={rel} lk={lk} nk={nk} E={E}", file=sys.stderr, flush=True)
+    with gpu.Structure(**st) as sc, gpu.Context(lk=lk, sites_per_pl=st["s"], n_pl=wl["P"]) as ctx:
         ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
-        ctx.set_structure_kkr(sc, wl["alpha"])
+        ctx.set_structure_kkr(sc, wl["alpha"], rel=rel)
         for i in range(args.warmup):
             ctx.tau_bz(E, k, w)
         times, stats = [], []
@@ -192,9 +193,9 @@ def kkr_bench(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic t^-1 (square wells), reference-form structure constants",
-        "config": {"workload": f"Nb(110) stack on the layer producer: lmax={st['lmax']} s={st['s']} "
-                               f"nextra={st['nextra']} P={wl['P']} n={st['s'] * (st['lmax'] + 1) ** 2} "
-                               f"E={E} 64x64 k",
+        "config": {"workload": f"Nb(110) stack on the layer producer: lmax={st['lmax']} "
+                               f"{'kappa-mu' if rel else 'lm'} lk={lk} s={st['s']} nextra={st['nextra']} "
+                               f"P={wl['P']} n={st['s'] * lk} E={E} 64x64 k",
                    "precision": "fp64", "nk": nk, "mean_iter_l": stl["mean_iter_l"],
                    "mean_iter_r": stl["mean_iter_r"],
                    "phase_ms": {"structure_constants": stl["ms_fourier"], "decimation": stl["ms_decimation"],
@@ -214,6 +215,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kkr", action="store_true",
                     help="k-loop on the layer structure-constant producer (Nb(110) stack)")
+    ap.add_argument("--kkr-rel", action="store_true",
+                    help="with --kkr: the relativistic lmax-3 stack of config 1's shape "
+                         "(lk 32, 4 sites/PL, 20 PLs) on the producer")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "

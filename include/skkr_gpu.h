@@ -215,7 +215,8 @@ int skkr_tau2d_batch(skkr_ctx *ctx, int nb, const double *m00,
  * |dz| * min|b| >= 5, screening G' = G (1 - alpha G)^-1 over (2 nextra + 3)
  * principal layers (scrstr.f), and the layer-tridiagonal blocks g2d.
  * Non-relativistic lm basis, nl2 = (lmax+1)^2 per layer; lmax <= 3 and
- * nl2 (2 nextra + 3) s <= 256.
+ * nl2 (2 nextra + 3) s <= 512 (above 256 rows 1 - alpha G is inverted by
+ * 2 x 2 blocks around the leading 256).
  * ------------------------------------------------------------------------ */
 typedef struct skkr_str skkr_str;
 

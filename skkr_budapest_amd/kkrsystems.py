@@ -51,13 +51,26 @@ def square_well_t(E, r, v, lmax):
     return np.array(out)
 
 
-def nb110_workload(E, P=3, lmax=2, geo=NB110):
+def clebsch_kappa_l(lmax):
+    """l of each kappa-mu index (2 (lmax+1)^2 of them) in the order of
+    clebsch.f, ir = 2 kap^2 + kap + m (+1): kap = -1, 1, -2, 2, -3, ..."""
+    order = []
+    for l in range(lmax + 1):
+        if l > 0:
+            order += [l] * (2 * l)        # kap = l, j = l - 1/2
+        order += [l] * (2 * l + 2)        # kap = -l - 1, j = l + 1/2
+    return order
+
+
+def nb110_workload(E, P=3, lmax=2, geo=NB110, rel=0):
     """Producer arguments, screening alpha[layer, l] and the screened t^-1 of
-    the interface (P*s sites) and of the leads (s sites), (lmax+1)^2 blocks."""
+    the interface (P*s sites) and of the leads (s sites): (lmax+1)^2 blocks,
+    or 2 (lmax+1)^2 kappa-mu blocks for rel (t^-1 depends on l only, so it
+    stays diagonal)."""
     s = geo["s"]
     al = np.array([square_well_t(E, geo["rmt"], 3.0, lmax) for _ in range(s)])
     tv = 1.0 / square_well_t(E, geo["rmt"], -1.0, lmax) - 1.0 / al[0]
-    lof = [l for l in range(lmax + 1) for _ in range(2 * l + 1)]
+    lof = clebsch_kappa_l(lmax) if rel else [l for l in range(lmax + 1) for _ in range(2 * l + 1)]
     ti = np.array([np.diag([tv[l] for l in lof]) for _ in range(s)])
     return dict(structure=dict(lmax=lmax, s=s, nextra=geo["nextra"], eta=geo["eta"], a1=geo["a1"],
                                a2=geo["a2"], cvec=np.array(geo["cvec"])),
