@@ -66,6 +66,17 @@ def build_fortran(verbose=False):
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
     if verbose:
         print("built", exe)
+    # the structure-constant producer called from Fortran
+    gsrc = os.path.join(FSRC, "skkr_gstore_main.f90")
+    if os.path.exists(gsrc):
+        gexe = os.path.join(FSRC, "skkr_gstore")
+        cmd = [fc, "-O2", "-J", bdir, srcs[0], gsrc, "-o", gexe,
+               "-L" + HERE, "-lskkr_gpu", "-Wl,-rpath," + HERE]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+        if verbose:
+            print("built", gexe)
     return exe
 
 

@@ -50,3 +50,29 @@ def test_fortran_kloop_matches_oracle(tmp_path, nprocs):
     ref = oracle.tau_bz(sy, E, k, w)
     assert np.linalg.norm(tau - ref["tau_bz"]) < 1e-11 * np.linalg.norm(ref["tau_bz"])
     assert np.abs(tkd - ref["trk"]).max() < 1e-11 * np.abs(ref["trk"]).max()
+
+
+GEXE = os.path.join(ROOT, "skkr_budapest_amd", "fortran", "skkr_gstore")
+
+
+@pytest.mark.gpu
+def test_fortran_gstore_matches_reference_golden(tmp_path):
+    """The producer from Fortran (skkr_gstore_main.f90 through the skkr_gpu
+    module) against g2d of the reference's own gstore."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", "gstore.npz"), allow_pickle=False)
+    p = "nb110_"
+    s, lmax = int(g[p + "s"]), int(g["lmax"])
+    E = complex(g[p + "E"])
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([lmax, s, int(g[p + "nextra"]), len(g[p + "ks"])], np.int32).tofile(f)
+        np.array([float(g[p + "eta"]), *g[p + "a1"], *g[p + "a2"], E.real, E.imag]).tofile(f)
+        np.ascontiguousarray(g[p + "cvec"], np.float64).tofile(f)
+        np.ascontiguousarray(g[p + "alpha_l"], np.complex128).tofile(f)
+        np.ascontiguousarray(g[p + "ks"], np.float64).tofile(f)
+    subprocess.run([GEXE, str(fin), str(fout)], check=True, timeout=120)
+    nl2 = (lmax + 1) ** 2
+    out = np.fromfile(fout, np.complex128).reshape(len(g[p + "ks"]), 3, s, 2 * s + 1, nl2, nl2)
+    for ik in range(len(g[p + "ks"])):
+        ref = g[p + "g2d_L"][ik]
+        assert np.abs(out[ik] - ref).max() < 1e-10 * np.abs(ref).max()
