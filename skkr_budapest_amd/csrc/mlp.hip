@@ -203,6 +203,7 @@ struct skkr_mlp {
 
 static int mlp_reserve(skkr_mlp* M, int n) {
   if (n <= M->cap) return 0;
+  M->cap = 0;  // until every buffer of the new size exists
   float** bufs[] = {&M->x, &M->t, &M->y, &M->h1, &M->h2, &M->d1, &M->d2, &M->d3, &M->part};
   for (float** b : bufs)
     if (*b) {

@@ -100,6 +100,7 @@ struct skkr_ctx {
   long long launches = 0;
   int timing_detail = 0;
   std::vector<hipEvent_t> evpool;  // pairs bracketing each dominant-kernel launch
+  hipEvent_t phase_ev[5] = {};     // phase boundaries of a tau_bz call, tile timing
   std::vector<double> evflops, evbytes;
   size_t evused = 0;
   double dom_ms = 0.0, dom_flops = 0.0, dom_bytes = 0.0;
@@ -302,6 +303,14 @@ static int ensure_inverse(skkr_ctx* c, int B) {
   CHK(fill(&c->p_Csc, [&](size_t j) { return c->Csc + j * h1 * h2; }, B2));
   CHK(fill(&c->p_Dsc, [&](size_t j) { return c->Dsc + j * h1 * h2; }, B2));
   c->invB = B;
+  return SKKR_OK;
+}
+
+// The context's phase events, created on first use and kept until
+// skkr_destroy, so that no return path leaks them.
+static int ensure_phase_events(skkr_ctx* c) {
+  for (hipEvent_t& e : c->phase_ev)
+    if (!e) HIPCHK(hipEventCreate(&e));
   return SKKR_OK;
 }
 
@@ -751,6 +760,8 @@ void skkr_destroy(skkr_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+  for (hipEvent_t e : c->phase_ev)
+    if (e) hipEventDestroy(e);
   delete c;
 }
 
@@ -922,8 +933,8 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   c->evbytes.clear();
   c->launches = 0;
   HIPCHK(hipMemsetAsync(c->taubz, 0, sizeof(z) * tsz, c->stream));
-  hipEvent_t ev[5];
-  for (auto& e : ev) hipEventCreate(&e);
+  CHK(ensure_phase_events(c));
+  hipEvent_t* ev = c->phase_ev;
   double ms_f = 0, ms_d = 0, ms_s = 0, ms_a = 0;
   double sum_l = 0, sum_r = 0;
   int maxit = 0, nbatches = 0;
@@ -1052,7 +1063,6 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   for (size_t e = 0; e < 2 * tsz; ++e) tau_bz[e] += hbz[e];
   if (phys)
     for (size_t e = 0; e < 2 * tsz; ++e) tau_bz_phys[e] += hph[e];
-  for (auto& e : ev) hipEventDestroy(e);
   if (st) {
     st->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     st->ms_fourier = ms_f;
@@ -1148,12 +1158,16 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
 // mean launch time (us) of the plain GEMM m x n x k over nb batch entries
 static int time_gemm(skkr_ctx* c, int tile, int m, int n, int k, z** A, int lda, z** B, int ldb,
                      z** C, int ldc, int nb, double* us) {
-  const int saved = c->gemm_tile_force;
+  CHK(ensure_phase_events(c));
+  // the forced tile is restored on every return path
+  struct Restore {
+    skkr_ctx* c;
+    int saved;
+    ~Restore() { c->gemm_tile_force = saved; }
+  } restore{c, c->gemm_tile_force};
   c->gemm_tile_force = tile;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  hipEvent_t e0 = c->phase_ev[0], e1 = c->phase_ev[1];
   int r = launch_zgemm(c, m, n, k, A, lda, B, ldb, C, ldc, one, zero, nb, nullptr);
   const int reps = 3;
   HIPCHK(hipEventRecord(e0, c->stream));
@@ -1163,9 +1177,6 @@ static int time_gemm(skkr_ctx* c, int tile, int m, int n, int k, z** A, int lda,
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  c->gemm_tile_force = saved;
   *us = 1e3 * ms / reps;
   return r;
 }
@@ -1236,14 +1247,23 @@ struct TmpMats {
   }
 };
 
+// temporary device buffer, freed on every return path
+struct TmpBuf {
+  void* d = nullptr;
+  ~TmpBuf() {
+    if (d) hipFree(d);
+  }
+};
+
 int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
   if (!c || n <= 0 || n > 256 || nb <= 0 || !a) return SKKR_ERR_ARG;
   if (n > 128 && n != c->n) return SKKR_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->cfg.device));
-  TmpMats m, st;
+  TmpMats m;
   CHK(m.init(nb, (size_t)n * n, a, c->stream));
-  int* dst = nullptr;
-  HIPCHK(hipMalloc(&dst, sizeof(int) * nb));
+  TmpBuf sbuf;
+  if (hipMalloc(&sbuf.d, sizeof(int) * nb) != hipSuccess) return SKKR_ERR_NOMEM;
+  int* dst = (int*)sbuf.d;
   HIPCHK(hipMemsetAsync(dst, 0, sizeof(int) * nb, c->stream));
   InvArgs ia;
   memset(&ia, 0, sizeof(ia));
@@ -1265,7 +1285,6 @@ int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
       if (code != SKKR_OK && r == SKKR_OK) r = code;
     }
   }
-  hipFree(dst);
   return r;
 }
 
