@@ -205,6 +205,20 @@ int skkr_tau2d_batch(skkr_ctx *ctx, int nb, const double *m00,
                      const double *m01, const double *m10, const double *x,
                      const double *y, double w, double *tau, int *status);
 
+/* tau2d for a layered stack (bulkgeo = .false., skkr_kpar_v6/main/tau2d.f:55-56,
+ * 90-91): every principal layer p has its own packer blocks of its own g2d
+ * plane, m00/m01/m10 (nb x P x n x n, k-major: block (k, p) at
+ * (k * P + p) * n * n), as tau2d's packer calls produce them for
+ * ipl = p + 1.  The matrix is block-tridiagonal with A_pp = m00_p + t^-1_p,
+ * A_p,p+1 = m01_p and A_p+1,p = m10_p+1; m10 of the first and m01 of the last
+ * PL are not read (the lead terms are y and x).  Equal sites per PL, as for
+ * skkr_tau2d_batch; the other arguments and tau are those of
+ * skkr_tau2d_batch. */
+int skkr_tau2d_layered_batch(skkr_ctx *ctx, int nb, const double *m00,
+                             const double *m01, const double *m10,
+                             const double *x, const double *y, double w,
+                             double *tau, int *status);
+
 /* ------------------------------------------------------------------------
  * Layer structure constants (the G(k) producer of the reference):
  * skkr_kpar_v6/main/gstore.f:6-7

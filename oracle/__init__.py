@@ -47,13 +47,14 @@ def lib():
         L.orc_surfgfd.argtypes = [P, P, P, P, i, i, i, d, P]
         L.orc_dugo.argtypes = [P, P, P, P, i, i, ctypes.c_char, i, d, P, P]
         L.orc_tau2d.argtypes = [P, P, P, P, i, i, i, P, P, d, P]
+        L.orc_tau2d_pl.argtypes = [P, P, P, ctypes.c_size_t, P, i, i, i, P, P, d, P]
         L.orc_fourier.argtypes = [i, P, P, i, P, P]
         L.orc_pl_blocks.argtypes = [P, P, i, P, P, P]
         L.orc_tau_bz.argtypes = [i, i, i, i, P, P, P, P, P, P, i, d, i, P, P, i,
                                  P, P, P, P, P]
         L.orc_phystau.argtypes = [i, P, P, P, P, i, i]
         L.orc_phystau.restype = None
-        for f in (L.orc_gjinv, L.orc_surfgfd, L.orc_dugo, L.orc_tau2d, L.orc_tau_bz):
+        for f in (L.orc_gjinv, L.orc_surfgfd, L.orc_dugo, L.orc_tau2d, L.orc_tau2d_pl, L.orc_tau_bz):
             f.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -135,6 +136,20 @@ def tau2d(m00g, m01, m10, tinv, P, x, y, w=1.0):
     tau = np.zeros((P * s, lk, lk), np.complex128)
     st = lib().orc_tau2d(_p(a), _p(b), _p(c), _p(t), lk, s, P, _p(xb), _p(yb),
                          float(w), _p(tau))
+    return _uncm(tau), st
+
+
+def tau2d_layered(m00g, m01, m10, tinv, x, y, w=1.0):
+    """tau2d.f:2-152 with bulkgeo false: PL p reads its own packer blocks
+    m00g[p], m01[p], m10[p] (plane ipl = p + 1, :55-56,90-91)."""
+    P, n = m00g.shape[0], m00g.shape[1]
+    lk = tinv.shape[1]
+    s = tinv.shape[0] // P
+    a, b, c, t = _cm(m00g), _cm(m01), _cm(m10), _cm(tinv)
+    xb, yb = _cm(x), _cm(y)
+    tau = np.zeros((P * s, lk, lk), np.complex128)
+    st = lib().orc_tau2d_pl(_p(a), _p(b), _p(c), n * n, _p(t), lk, s, P, _p(xb), _p(yb),
+                            float(w), _p(tau))
     return _uncm(tau), st
 
 

@@ -17,6 +17,12 @@ c  Output: complex*16 taubz(lk,lk,P*s), trk(P*s,nk), tausave(lk,lk,P*s,nsave).
 c  Decimation iterations are printed by surfgfd (ichk=1) on stdout.
 c  -DGOD (with -DDBOGO): tau2dgod of skkr_dbogo_bsf/clu/tau2d_god.f instead
 c  of tau2d, and tausave holds the full tau(lk,lk,P*s,P*s) of the saved k.
+c  -DLAYERED (with -DDBOGO, whose param.h has mprc1 = mprc): a layered
+c  stack, tau2d with bulkgeo=.false. (tau2d.f:55-56,90-91).  Per k, after
+c  gintra, ginter (the bulk leads, as gg2dbulk of nrsp-scf/cpacoord.f:216-
+c  262), P intra blocks gip(n,n) and P-1 couplings gop(n,n) of PL p to PL
+c  p+1; plane ipl of g2d holds PL ipl's rows (its couplings to the leads
+c  are the bulk ginter).
       program refdrv
       implicit real*8 (a-h,o-z)
       include 'param.h'
@@ -26,6 +32,10 @@ c  of tau2d, and tausave holds the full tau(lk,lk,P*s,P*s) of the saved k.
       parameter(mdim=mdimr, lgd=lmmaxp)
 #endif
       complex*16 g2d(lgd,lgd,-minprc:minprc,minprc,0:mprc1+1)
+#ifdef LAYERED
+      complex*16 g2db(lgd,lgd,-minprc:minprc,minprc,0:mprc1+1)
+      complex*16, allocatable :: gip(:,:,:),gop(:,:,:)
+#endif
       complex*16 xx(mdim,mdim),yy(mdim,mdim)
       complex*16, allocatable :: tinv(:,:,:),tinvl(:,:,:),tinvr(:,:,:)
       complex*16, allocatable :: tau1(:,:,:),taubz(:,:,:)
@@ -66,6 +76,9 @@ c
       allocate(tau1(lk,lk,mintfc),taubz(lk,lk,np*ns))
       allocate(trk(np*ns,nk),tsave(lk,lk,np*ns,max(nsave,1)))
       allocate(gi(n,n),go(n,n),w(nk))
+#ifdef LAYERED
+      allocate(gip(n,n,np),gop(n,n,max(np-1,1)))
+#endif
 #ifdef GOD
       allocate(tauf(lk,lk,np*ns,np*ns))
       allocate(tfsave(lk,lk,np*ns,np*ns,max(nsave,1)))
@@ -98,43 +111,49 @@ c
         read(11) go
         g2d=(0.d0,0.d0)
         do ipl=0,2
-        do i=1,ns
-c         coupling to PL p-1: offsets -s..-i, column site jp=s+i+j
-          do j=-ns,-i
-            jp=ns+i+j
-            do ia=1,lk
-            do ib=1,lk
-              g2d(ia,ib,j,i,ipl)=dconjg(go((jp-1)*lk+ib,(i-1)*lk+ia))
-            end do
-            end do
-          end do
-c         within the PL: offsets 1-i..s-i, column site jp=i+j
-          do j=1-i,ns-i
-            jp=i+j
-            do ia=1,lk
-            do ib=1,lk
-              g2d(ia,ib,j,i,ipl)=gi((i-1)*lk+ia,(jp-1)*lk+ib)
-            end do
-            end do
-          end do
-c         coupling to PL p+1: offsets s-i+1..s, column site jp=j+i-s
-          do j=ns-i+1,ns
-            jp=j+i-ns
-            do ia=1,lk
-            do ib=1,lk
-              g2d(ia,ib,j,i,ipl)=go((i-1)*lk+ia,(jp-1)*lk+ib)
-            end do
-            end do
-          end do
+          call fillpl(g2d(1,1,-minprc,1,ipl),gi,go,go,lk,ns,n,lgd,
+     &                minprc)
         end do
+#ifdef LAYERED
+        g2db=g2d
+        do ipl=1,np
+          read(11) gip(:,:,ipl)
         end do
+        do ipl=1,np-1
+          read(11) gop(:,:,ipl)
+        end do
+        do ipl=1,np
+          if(ipl.eq.1.and.ipl.eq.np) then
+            call fillpl(g2d(1,1,-minprc,1,ipl),gip(1,1,ipl),go,go,
+     &                  lk,ns,n,lgd,minprc)
+          else if(ipl.eq.1) then
+            call fillpl(g2d(1,1,-minprc,1,ipl),gip(1,1,ipl),go,
+     &                  gop(1,1,ipl),lk,ns,n,lgd,minprc)
+          else if(ipl.eq.np) then
+            call fillpl(g2d(1,1,-minprc,1,ipl),gip(1,1,ipl),
+     &                  gop(1,1,ipl-1),go,lk,ns,n,lgd,minprc)
+          else
+            call fillpl(g2d(1,1,-minprc,1,ipl),gip(1,1,ipl),
+     &                  gop(1,1,ipl-1),gop(1,1,ipl),lk,ns,n,lgd,minprc)
+          end if
+        end do
+#endif
 c
+#ifdef LAYERED
+        call dugo(lmax,lk,tinvl,lk,g2db(1,1,-minprc,1,0),
+     &            g2db(1,1,-minprc,1,1),yy,mdim,'L',.true.,0,sgfconv)
+        if(.not.sgfconv) stop 3
+        call dugo(lmax,lk,tinvr,lk,g2db(1,1,-minprc,1,2),
+     &            g2db(1,1,-minprc,1,1),xx,mdim,'R',.true.,0,sgfconv)
+        if(.not.sgfconv) stop 3
+#else
         call dugo(lmax,lk,tinvl,lk,g2d(1,1,-minprc,1,0),
      &            g2d(1,1,-minprc,1,1),yy,mdim,'L',.true.,0,sgfconv)
         if(.not.sgfconv) stop 3
         call dugo(lmax,lk,tinvr,lk,g2d(1,1,-minprc,1,2),
      &            g2d(1,1,-minprc,1,1),xx,mdim,'R',.true.,0,sgfconv)
         if(.not.sgfconv) stop 3
+#endif
 #ifdef GOD
         call tau2dgod(0,1.d0,.true.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
      &                tauf,mdim)
@@ -144,6 +163,9 @@ c
         do is=1,nsave
           if(isave(is).eq.ik) tfsave(:,:,:,:,is)=tauf
         end do
+#elif defined(LAYERED)
+        call tau2d(0,1.d0,.false.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
+     &             tau1,mdim)
 #else
         call tau2d(0,1.d0,.true.,lmax,lk,lk,np*ns,tinv,g2d,xx,yy,
      &             tau1,mdim)
@@ -175,4 +197,40 @@ c
       if(nsave.gt.0) write(12) tsave(:,:,:,1:nsave)
 #endif
       close(12)
+      end
+c
+c  Rows of one principal layer in g2d(lm,lm',jj,icount) (packer.f:36-95):
+c  row site i couples to PL p-1 through offsets -s..-i (gm, the coupling of
+c  PL p-1 to PL p, read transposed and conjugated), within PL p through
+c  1-i..s-i (gi) and to PL p+1 through s-i+1..s (gn).  G, not -G.
+      subroutine fillpl(g2p,gi,gm,gn,lk,ns,n,lgd,mnp)
+      implicit real*8 (a-h,o-z)
+      complex*16 g2p(lgd,lgd,-mnp:mnp,mnp)
+      complex*16 gi(n,n),gm(n,n),gn(n,n)
+      do i=1,ns
+        do j=-ns,-i
+          jp=ns+i+j
+          do ia=1,lk
+          do ib=1,lk
+            g2p(ia,ib,j,i)=dconjg(gm((jp-1)*lk+ib,(i-1)*lk+ia))
+          end do
+          end do
+        end do
+        do j=1-i,ns-i
+          jp=i+j
+          do ia=1,lk
+          do ib=1,lk
+            g2p(ia,ib,j,i)=gi((i-1)*lk+ia,(jp-1)*lk+ib)
+          end do
+          end do
+        end do
+        do j=ns-i+1,ns
+          jp=j+i-ns
+          do ia=1,lk
+          do ib=1,lk
+            g2p(ia,ib,j,i)=gn((i-1)*lk+ia,(jp-1)*lk+ib)
+          end do
+          end do
+        end do
+      end do
       end

@@ -14,6 +14,7 @@
  *   orc_surfgfd  skkr_kpar_v6/main/surfgf.f:2-108   decimation
  *   orc_dugo     skkr_kpar_v6/main/dugo.f:9-120     lead self-energy (bulk)
  *   orc_tau2d    skkr_kpar_v6/main/tau2d.f:2-152    Godfrin sweeps
+ *   orc_tau2d_pl the same with one packer plane per PL (bulkgeo false)
  *   orc_phystau  skkr_kpar_v6/nrsp-scf/phystau.f:2-58 (and rsp-scf/phystau.f,
  *                skkr_dbogo_bsf/rsp-scf/phystau_d.f): screened -> physical tau
  *   orc_fourier  G(k) = sum_R G_r(R) exp(-i k.R)    (no reference routine;
@@ -293,12 +294,14 @@ int orc_dugo(const cplx *m00g, const cplx *m01, const cplx *m10,
   return st;
 }
 
-/* tau2d.f:2-152 with bulkgeo (same G blocks for every PL).
+/* tau2d.f:2-152.  PL p (0-based) reads its packer blocks at m00g, m01, m10
+ * + p * pst: pst = 0 is bulkgeo (the same plane for every PL, :55-56,90-91),
+ * pst = n*n a layered stack (plane ipl = p + 1 of its own).
  * tinv: P*s site blocks; x = X_P (right lead), y = Y_1 (left lead);
  * tau: P*s site blocks, each multiplied by w (:137). */
-int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
-              const cplx *tinv, int lk, int s, int P, const cplx *x,
-              const cplx *y, double w, cplx *tau) {
+int orc_tau2d_pl(const cplx *m00g, const cplx *m01g, const cplx *m10g, size_t pst,
+                 const cplx *tinv, int lk, int s, int P, const cplx *x,
+                 const cplx *y, double w, cplx *tau) {
   int n = lk * s;
   size_t nn = (size_t)n * n;
   cplx *xx = (cplx *)malloc(sizeof(cplx) * nn * P);
@@ -312,7 +315,8 @@ int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
   memcpy(xx + nn * (P - 1), x, sizeof(cplx) * nn);
   /* Y sweep (:53-85) */
   for (int p = 0; p < P; ++p) {
-    memcpy(m00, m00g, sizeof(cplx) * nn);
+    const cplx *m01 = m01g + p * pst, *m10 = m10g + p * pst;
+    memcpy(m00, m00g + p * pst, sizeof(cplx) * nn);
     add_tinv(m00, n, tinv + (size_t)p * s * lk * lk, lk, s);
     memcpy(t00 + nn * p, m00, sizeof(cplx) * nn);
     if (p > 0) {
@@ -329,6 +333,7 @@ int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
   }
   /* X sweep (:89-109) */
   for (int p = P - 1; p >= 0; --p) {
+    const cplx *m01 = m01g + p * pst, *m10 = m10g + p * pst;
     if (p < P - 1) {
       orc_doubmt2(m01, xx + nn * p, m00, n, n, n, n);
       memcpy(xx + nn * p, m00, sizeof(cplx) * nn);
@@ -358,6 +363,13 @@ int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
 done:
   free(xx); free(yy); free(t00); free(m00); free(help);
   return status;
+}
+
+/* tau2d.f:2-152 with bulkgeo (same G blocks for every PL). */
+int orc_tau2d(const cplx *m00g, const cplx *m01, const cplx *m10,
+              const cplx *tinv, int lk, int s, int P, const cplx *x,
+              const cplx *y, double w, cplx *tau) {
+  return orc_tau2d_pl(m00g, m01, m10, 0, tinv, lk, s, P, x, y, w, tau);
 }
 
 /* G(k) = sum_R G_r(R) exp(-i k.R), R summed in index order. */

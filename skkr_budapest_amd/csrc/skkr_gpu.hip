@@ -96,6 +96,11 @@ struct skkr_ctx {
   PtrArr p_AB[2], p_alpha[2], p_beta[2];  // ping-pong [alpha; beta]
   PtrArr p_M00, p_M01, p_M10, p_W, p_Z, p_X, p_Y;
   std::vector<PtrArr> p_G, p_Zs, p_tau, p_tr;
+  // per-PL blocks of a layered (non-bulk) stack, device layout [p][k]
+  z *M00pl = nullptr, *M01pl = nullptr, *M10pl = nullptr;
+  int pl_cap = 0;
+  std::vector<PtrArr> p_M00pl, p_M01pl, p_M10pl;
+  bool layered = false;  // the sweeps read the per-PL blocks
   // stats
   long long launches = 0;
   int timing_detail = 0;
@@ -616,22 +621,27 @@ static int sweep_launches(skkr_ctx* c, int nb) {
   const int n = c->n, P = c->cfg.n_pl, s = c->cfg.sites_per_pl, lk = c->cfg.lk;
   const size_t tblk = (size_t)s * lk * lk;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // blocks of PL p: one set for a bulk-like stack, per PL for a layered one
+  // (A_pp = M00_p + t^-1_p, A_p,p+1 = M01_p, A_p+1,p = M10_p+1)
+  auto m00 = [&](int p) { return c->layered ? c->p_M00pl[p].d : c->p_M00.d; };
+  auto m01 = [&](int p) { return c->layered ? c->p_M01pl[p].d : c->p_M01.d; };
+  auto m10 = [&](int p) { return c->layered ? c->p_M10pl[p].d : c->p_M10.d; };
   for (int p = 0; p < P - 1; ++p) {
     InvArgs ia = inv_args(c);
-    ia.base = c->p_M00.d;
+    ia.base = m00(p);
     ia.tinv = c->dTint + p * tblk;
     ia.sub1 = c->p_Y.d;
     ia.out = c->p_G[p].d;
     ia.status = c->status_sw;
     CHK(run_inverse(c, ia, nb));
-    CHK(launch_zgemm(c, n, n, n, c->p_G[p].d, n, c->p_M01.d, n, c->p_Zs[p].d, n, one, zero, nb,
+    CHK(launch_zgemm(c, n, n, n, c->p_G[p].d, n, m01(p), n, c->p_Zs[p].d, n, one, zero, nb,
                      nullptr));
-    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_Zs[p].d, n, c->p_Y.d, n, one, zero, nb,
+    CHK(launch_zgemm(c, n, n, n, m10(p + 1), n, c->p_Zs[p].d, n, c->p_Y.d, n, one, zero, nb,
                      nullptr));
   }
   {
     InvArgs ia = inv_args(c);
-    ia.base = c->p_M00.d;
+    ia.base = m00(P - 1);
     ia.tinv = c->dTint + (P - 1) * tblk;
     ia.sub1 = c->p_X.d;
     ia.sub2 = c->p_Y.d;
@@ -642,7 +652,7 @@ static int sweep_launches(skkr_ctx* c, int nb) {
   }
   for (int p = P - 2; p >= 0; --p) {
     z** W = c->pairs_on ? c->p_Ws[p].d : c->p_W.d;
-    CHK(launch_zgemm(c, n, n, n, c->p_M10.d, n, c->p_G[p].d, n, W, n, one, zero, nb,
+    CHK(launch_zgemm(c, n, n, n, m10(p + 1), n, c->p_G[p].d, n, W, n, one, zero, nb,
                      nullptr));
     CHK(launch_zgemm(c, n, n, n, c->p_G[p + 1].d, n, W, n, c->p_Z.d, n, one, zero, nb,
                      nullptr));
@@ -656,7 +666,7 @@ static int sweep_launches(skkr_ctx* c, int nb) {
 // The sweeps of a batch, optionally as a captured HIP graph (one per nb; all
 // buffers and pointer arrays are fixed for the context's batch size).
 static int run_sweeps(skkr_ctx* c, int nb) {
-  if (!c->graphs || c->pairs_on) return sweep_launches(c, nb);
+  if (!c->graphs || c->pairs_on || c->layered) return sweep_launches(c, nb);
   auto it = c->sweep_graphs.find(nb);
   if (it == c->sweep_graphs.end()) {
     hipGraph_t g = nullptr;
@@ -836,6 +846,30 @@ static int ensure_ws(skkr_ctx* c) {
     CHK(make_ptrs(c, &c->p_Ws[p], v));
   }
   c->ws_cap = c->B;
+  return SKKR_OK;
+}
+
+// Device buffers of the per-PL blocks of a layered stack (P x B matrices each).
+static int ensure_layered(skkr_ctx* c) {
+  const int P = c->cfg.n_pl;
+  if (c->pl_cap >= c->B) return SKKR_OK;
+  const size_t nn = c->nn, Bs = c->B;
+  CHK(dalloc_t(c, &c->M00pl, (size_t)P * Bs * nn));
+  CHK(dalloc_t(c, &c->M01pl, (size_t)P * Bs * nn));
+  CHK(dalloc_t(c, &c->M10pl, (size_t)P * Bs * nn));
+  c->p_M00pl.assign(P, PtrArr());
+  c->p_M01pl.assign(P, PtrArr());
+  c->p_M10pl.assign(P, PtrArr());
+  for (int p = 0; p < P; ++p) {
+    std::vector<z*> v(Bs);
+    for (size_t j = 0; j < Bs; ++j) v[j] = c->M00pl + ((size_t)p * Bs + j) * nn;
+    CHK(make_ptrs(c, &c->p_M00pl[p], v));
+    for (size_t j = 0; j < Bs; ++j) v[j] = c->M01pl + ((size_t)p * Bs + j) * nn;
+    CHK(make_ptrs(c, &c->p_M01pl[p], v));
+    for (size_t j = 0; j < Bs; ++j) v[j] = c->M10pl + ((size_t)p * Bs + j) * nn;
+    CHK(make_ptrs(c, &c->p_M10pl[p], v));
+  }
+  c->pl_cap = c->B;
   return SKKR_OK;
 }
 
@@ -1376,6 +1410,50 @@ int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
   HIPCHK(hipMemcpyAsync(c->Ycur, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
   CHK(run_sweeps(c, nb));
+  long long ne = (long long)tsz * nb;
+  scale_copy_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(ne, c->tauk, w, c->tauk);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tau, c->tauk, sizeof(z) * tsz * nb, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> hs(nb);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->status_sw, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = SKKR_OK;
+  for (int j = 0; j < nb; ++j) {
+    int code = status_code(hs[j]);
+    if (status) status[j] = code;
+    if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+  }
+  return rc;
+}
+
+int skkr_tau2d_layered_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
+                             const double* m10, const double* x, const double* y, double w,
+                             double* tau, int* status) {
+  if (!c || nb <= 0 || !m00 || !m01 || !m10 || !x || !y || !tau || !c->have_tinv)
+    return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  CHK(ensure_batch(c, std::max(c->B, nb)));
+  CHK(ensure_layered(c));
+  const size_t nn = c->nn, Bs = c->B;
+  const int P = c->cfg.n_pl, lk = c->cfg.lk;
+  const size_t tsz = (size_t)P * c->ns * lk * lk;
+  // host [k][p] blocks -> device [p][k]
+  const size_t blk = sizeof(z) * nn;
+  for (int p = 0; p < P; ++p) {
+    HIPCHK(hipMemcpy2DAsync(c->M00pl + (size_t)p * Bs * nn, blk, m00 + 2 * (size_t)p * nn, blk * P, blk, nb,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpy2DAsync(c->M01pl + (size_t)p * Bs * nn, blk, m01 + 2 * (size_t)p * nn, blk * P, blk, nb,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpy2DAsync(c->M10pl + (size_t)p * Bs * nn, blk, m10 + 2 * (size_t)p * nn, blk * P, blk, nb,
+                            hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(hipMemcpyAsync(c->Xc, x, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->Ycur, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
+  c->layered = true;
+  const int rs = run_sweeps(c, nb);
+  c->layered = false;
+  CHK(rs);
   long long ne = (long long)tsz * nb;
   scale_copy_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(ne, c->tauk, w, c->tauk);
   HIPCHK(hipGetLastError());

@@ -180,5 +180,30 @@ module skkr_gpu
        complex(c_double_complex), intent(inout) :: tau_bz(*), tau_ij(*), tau_ji(*)
        type(c_ptr), value :: iters_k, status_k, stats
      end function skkr_tau_bz_pairs
+
+     ! tau2d (main/tau2d.f:2) for nb k-points: bulkgeo (one set of packer
+     ! blocks m00(n,n,nb)) and layered (m00(n,n,P,nb), one set per PL,
+     ! tau2d.f:55-56,90-91); tau(lk,lk,P*s,nb) = w * site-diagonal blocks
+     integer(c_int) function skkr_tau2d_batch(ctx, nb, m00, m01, m10, x, y, w, tau, status) &
+          bind(C, name="skkr_tau2d_batch")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nb
+       complex(c_double_complex), intent(in) :: m00(*), m01(*), m10(*), x(*), y(*)
+       real(c_double), value :: w
+       complex(c_double_complex), intent(out) :: tau(*)
+       integer(c_int), intent(out) :: status(*)
+     end function skkr_tau2d_batch
+
+     integer(c_int) function skkr_tau2d_layered_batch(ctx, nb, m00, m01, m10, x, y, w, tau, &
+          status) bind(C, name="skkr_tau2d_layered_batch")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nb
+       complex(c_double_complex), intent(in) :: m00(*), m01(*), m10(*), x(*), y(*)
+       real(c_double), value :: w
+       complex(c_double_complex), intent(out) :: tau(*)
+       integer(c_int), intent(out) :: status(*)
+     end function skkr_tau2d_layered_batch
   end interface
 end module skkr_gpu

@@ -35,7 +35,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
            "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs",
-           "skkr_set_structure_kkr_rel")
+           "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch")
 
 
 class SkkrError(RuntimeError):
@@ -102,6 +102,7 @@ def lib():
         L.skkr_surfgfd_batch.argtypes = [V, I, V, V, V, V, V, V]
         L.skkr_dugo_batch.argtypes = [V, ctypes.c_char, I, V, V, V, V, V, V]
         L.skkr_tau2d_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
+        L.skkr_tau2d_layered_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
         L.skkr_set_option.argtypes = [V, ctypes.c_char_p, I]
         L.skkr_autotune.argtypes = [V, ctypes.POINTER(Tuning)]
         L.skkr_set_screening.argtypes = [V, I, V, V]
@@ -376,6 +377,23 @@ class Context:
         tb = colmajor(tau)
         status = np.zeros(nb, np.int32)
         lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
+        return from_colmajor(tb), status
+
+    def tau2d_layered(self, m00, m01, m10, x, y, w=1.0):
+        """tau2d of a layered stack (bulkgeo false): m00, m01, m10 are
+        (nb, P, n, n), the packer blocks of every PL's own g2d plane; x, y
+        (nb, n, n) the lead terms.  Returns (tau (nb, P*s, lk, lk), status);
+        raises on an argument error."""
+        nb = m00.shape[0]
+        if m00.shape[:2] != (nb, self.P) or m01.shape != m00.shape or m10.shape != m00.shape:
+            raise ValueError("m00, m01, m10 must be (nb, P, n, n)")
+        bufs = [colmajor(_c128(m)) for m in (m00, m01, m10, x, y)]
+        tau = np.zeros((nb, self.P * self.s, self.lk, self.lk), np.complex128)
+        tb = colmajor(tau)
+        status = np.zeros(nb, np.int32)
+        rc = lib().skkr_tau2d_layered_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
+        if rc == 1:
+            self._check(rc, "skkr_tau2d_layered_batch")
         return from_colmajor(tb), status
 
 
