@@ -66,10 +66,12 @@ def build_fortran(verbose=False):
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
     if verbose:
         print("built", exe)
-    # the structure-constant producer called from Fortran
-    gsrc = os.path.join(FSRC, "skkr_gstore_main.f90")
-    if os.path.exists(gsrc):
-        gexe = os.path.join(FSRC, "skkr_gstore")
+    # the structure-constant producer and tau2d called from Fortran
+    for src, name in (("skkr_gstore_main.f90", "skkr_gstore"), ("skkr_tau2d_main.f90", "skkr_tau2d")):
+        gsrc = os.path.join(FSRC, src)
+        if not os.path.exists(gsrc):
+            continue
+        gexe = os.path.join(FSRC, name)
         cmd = [fc, "-O2", "-J", bdir, srcs[0], gsrc, "-o", gexe,
                "-L" + HERE, "-lskkr_gpu", "-Wl,-rpath," + HERE]
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -76,3 +76,52 @@ def test_fortran_gstore_matches_reference_golden(tmp_path):
     for ik in range(len(g[p + "ks"])):
         ref = g[p + "g2d_L"][ik]
         assert np.abs(out[ik] - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+TEXE = os.path.join(ROOT, "skkr_budapest_amd", "fortran", "skkr_tau2d")
+
+
+def _run_tau2d_host(tmp_path, cfg, layered, tinv3, m00, m01, m10, x, y, w):
+    nb, lk, ns = m00.shape[0], cfg.lk, cfg.P * cfg.s
+    fin, fout = tmp_path / "t_in.bin", tmp_path / "t_out.bin"
+    with open(fin, "wb") as f:
+        np.array([lk, cfg.s, cfg.P, nb, int(layered)], np.int32).tofile(f)
+        for a in (*tinv3, m00, m01, m10, x, y):
+            S.colmajor(a).tofile(f)
+        np.array([w], np.float64).tofile(f)
+    subprocess.run([TEXE, str(fin), str(fout)], check=True, timeout=120)
+    raw = np.fromfile(fout, np.uint8)
+    nt = nb * ns * lk * lk * 16
+    tau = S.from_colmajor(np.frombuffer(raw[:nt], np.complex128).reshape(nb, ns, lk, lk))
+    return tau, np.frombuffer(raw[nt:], np.int32)
+
+
+@pytest.mark.gpu
+def test_fortran_tau2d_layered_matches_reference_golden(tmp_path):
+    """tau2d replaced from Fortran (skkr_tau2d_main.f90 through the skkr_gpu
+    module) on the layered stack, against the reference's own tau2d with
+    bulkgeo false; the bulk form against the oracle's bulkgeo tau2d."""
+    from test_gpu_layered import _inputs
+    from test_layered_oracle import LG
+    g = np.load(os.path.join(ROOT, "tests", "golden", "layered16.npz"), allow_pickle=False)
+    sy, pls = LG.layered_system()
+    E = complex(g["E"])
+    tinv3, m00, m01, m10, X, Y = _inputs(sy, pls, g["k"], E)
+    tau, st = _run_tau2d_host(tmp_path, sy.cfg, True, tinv3, m00, m01, m10, X, Y, 1.0)
+    assert (st == 0).all()
+    for j, ik in enumerate(g["save_idx"]):
+        ref = g["tau_save"][j]
+        assert np.linalg.norm(tau[ik] - ref) < 1e-11 * np.linalg.norm(ref)
+    bz = np.tensordot(g["w"], tau, axes=(0, 0))
+    assert np.linalg.norm(bz - g["tau_bz"]) < 1e-11 * np.linalg.norm(g["tau_bz"])
+    # bulkgeo: one plane (the leads' structure) for every PL, weight 0.5
+    ti, tl, tr = tinv3
+    b = [oracle.pl_blocks(*LG.gk(sy, kk)) for kk in g["k"][:3]]
+    b00, b01, b10 = (np.stack([bb[i] for bb in b]) for i in range(3))
+    bx = np.stack([oracle.dugo(*bb, tr, "R")[0] for bb in b])
+    by = np.stack([oracle.dugo(*bb, tl, "L")[0] for bb in b])
+    tau, st = _run_tau2d_host(tmp_path, sy.cfg, False, tinv3, b00, b01, b10, bx, by, 0.5)
+    assert (st == 0).all()
+    for i in range(3):
+        ref, _ = oracle.tau2d(b00[i], b01[i], b10[i], ti, sy.cfg.P, bx[i], by[i], w=0.5)
+        assert np.linalg.norm(tau[i] - ref) < 1e-11 * np.linalg.norm(ref)
