@@ -219,6 +219,18 @@ int skkr_tau2d_layered_batch(skkr_ctx *ctx, int nb, const double *m00,
                              const double *x, const double *y, double w,
                              double *tau, int *status);
 
+/* Point group of an irreducible-wedge k-mesh (intbz = 1,
+ * skkr_kpar_v6/nrsp-scf/cpacoord.f:98-101,281-299): the BZ sums of
+ * skkr_tau_bz, skkr_tau_bz_phys and skkr_cpa_bz become
+ * sum_k (w_k / ng) sum_g dg_g tau_k dg1_g per site, with the ng group
+ * matrices dg and their inverses dg1 (ng x lk x lk each, the sphbas output in
+ * the context's site basis).  The per-k traces are left as they are (the
+ * group does not mix l).  ng = 0 switches it off; ng <= 48.  Pair sums
+ * (skkr_tau_bz_pairs) need the full BZ and return SKKR_ERR_UNSUPPORTED while
+ * a group with ng > 1 is set. */
+int skkr_set_point_group(skkr_ctx *ctx, int ng, const double *dg,
+                         const double *dg1);
+
 /* ------------------------------------------------------------------------
  * Layer structure constants (the G(k) producer of the reference):
  * skkr_kpar_v6/main/gstore.f:6-7

@@ -139,6 +139,17 @@ def tau2d(m00g, m01, m10, tinv, P, x, y, w=1.0):
     return _uncm(tau), st
 
 
+def symmetrize(tau_bz, dg, dg1):
+    """The point-group sum of nrsp-scf/cpacoord.f:294-299 (tripmt, addmatc):
+    (1/ng) sum_g dg_g tau dg1_g per site, here applied to the BZ sum, which
+    the per-k form equals by linearity."""
+    tau_bz = np.asarray(tau_bz, np.complex128)
+    out = np.zeros_like(tau_bz)
+    for d, d1 in zip(dg, dg1):
+        out += d[None] @ tau_bz @ d1[None]
+    return out / len(dg)
+
+
 def tau2d_layered(m00g, m01, m10, tinv, x, y, w=1.0):
     """tau2d.f:2-152 with bulkgeo false: PL p reads its own packer blocks
     m00g[p], m01[p], m10[p] (plane ipl = p + 1, :55-56,90-91)."""

@@ -17,6 +17,10 @@ c  Output: complex*16 taubz(lk,lk,P*s), trk(P*s,nk), tausave(lk,lk,P*s,nsave).
 c  Decimation iterations are printed by surfgfd (ichk=1) on stdout.
 c  -DGOD (with -DDBOGO): tau2dgod of skkr_dbogo_bsf/clu/tau2d_god.f instead
 c  of tau2d, and tausave holds the full tau(lk,lk,P*s,P*s) of the saved k.
+c  -DSYM: the BZ sum of an irreducible wedge (intbz=1) as
+c  nrsp-scf/cpacoord.f:281-299 forms it: after w(nk) the input holds
+c  int32 ng; complex*16 dg(lk,lk,ng), dg1(lk,lk,ng) (sphbas), and every
+c  k adds (w/ng) dg tau dg1 of each group element (tripmt, addmatc).
 c  -DLAYERED (with -DDBOGO, whose param.h has mprc1 = mprc): a layered
 c  stack, tau2d with bulkgeo=.false. (tau2d.f:55-56,90-91).  Per k, after
 c  gintra, ginter (the bulk leads, as gg2dbulk of nrsp-scf/cpacoord.f:216-
@@ -44,6 +48,10 @@ c  are the bulk ginter).
       complex*16, allocatable :: tauf(:,:,:,:),tfsave(:,:,:,:,:)
 #endif
       complex*16, allocatable :: gi(:,:),go(:,:)
+#ifdef SYM
+      complex*16, allocatable :: dg(:,:,:),dg1(:,:,:),tsym(:,:)
+      integer ng
+#endif
       real*8, allocatable :: w(:)
       integer, allocatable :: isave(:)
       integer lk,ns,np,nk,itmx,nsave
@@ -90,6 +98,12 @@ c
       read(11) tinvl(:,:,1:ns)
       read(11) tinvr(:,:,1:ns)
       read(11) w
+#ifdef SYM
+      read(11) ng
+      allocate(dg(lk,lk,ng),dg1(lk,lk,ng),tsym(lk,lk))
+      read(11) dg
+      read(11) dg1
+#endif
 c
       itest=0
       errmax=errm
@@ -175,11 +189,19 @@ c
           do l=1,lk
             trk(li,ik)=trk(li,ik)+tau1(l,l,li)
           end do
+#ifdef SYM
+          do ig=1,ng
+            tsym=tau1(:,:,li)
+            call tripmt(dg(1,1,ig),tsym,dg1(1,1,ig),lk,lk,lk)
+            call addmatc(taubz(1,1,li),tsym,dcmplx(w(ik)/ng,0.d0),lk,lk)
+          end do
+#else
           do l=1,lk
           do lp=1,lk
             taubz(l,lp,li)=taubz(l,lp,li)+w(ik)*tau1(l,lp,li)
           end do
           end do
+#endif
         end do
         do is=1,nsave
           if(isave(is).eq.ik) tsave(:,:,:,is)=tau1(:,:,1:np*ns)

@@ -101,6 +101,11 @@ struct skkr_ctx {
   int pl_cap = 0;
   std::vector<PtrArr> p_M00pl, p_M01pl, p_M10pl;
   bool layered = false;  // the sweeps read the per-PL blocks
+  // point group of an irreducible-wedge mesh: tau_BZ <- (1/ng) sum_g dg tau_BZ dg1
+  int ng_sym = 0, sym_cap = 0;
+  z *dSym = nullptr, *dSymOut = nullptr, *dSymTmp = nullptr;  // dg (ng), dg1 (ng); sites each
+  PtrArr p_symT, p_symOut, p_symTmp;
+  std::vector<PtrArr> p_symD, p_symD1;  // dg_g, dg1_g repeated for every site
   // stats
   long long launches = 0;
   int timing_detail = 0;
@@ -941,6 +946,32 @@ static int ensure_mt(skkr_ctx* c) {
   return SKKR_OK;
 }
 
+// The point-group sum of an irreducible-wedge BZ integration,
+// nrsp-scf/cpacoord.f:294-299: every k adds (w_k/ng) dg_g tau_k dg1_g for
+// every group element.  Applied once to the BZ sum on the device (equal by
+// linearity): per element two batched products over the sites,
+// tmp = tau dg1_g and out += (1/ng) dg_g tmp; then tau <- out.
+static int apply_point_group(skkr_ctx* c) {
+  const int lk = c->cfg.lk, S = (int)(c->cfg.n_pl * c->ns);
+  const z zero = make_double2(0.0, 0.0), one = make_double2(1.0, 0.0);
+  const z scale = make_double2(1.0 / c->ng_sym, 0.0);
+  const int td = c->timing_detail;
+  c->timing_detail = 0;  // not part of the dominant-kernel statistics
+  int r = SKKR_OK;
+  for (int g = 0; g < c->ng_sym && r == SKKR_OK; ++g) {
+    r = launch_zgemm(c, lk, lk, lk, c->p_symT.d, lk, c->p_symD1[g].d, lk, c->p_symTmp.d, lk, one, zero, S,
+                     nullptr);
+    if (r == SKKR_OK)
+      r = launch_zgemm(c, lk, lk, lk, c->p_symD[g].d, lk, c->p_symTmp.d, lk, c->p_symOut.d, lk, scale,
+                       g == 0 ? zero : one, S, nullptr);
+  }
+  c->timing_detail = td;
+  CHK(r);
+  HIPCHK(hipMemcpyAsync(c->taubz, c->dSymOut, sizeof(z) * (size_t)S * lk * lk, hipMemcpyDeviceToDevice,
+                        c->stream));
+  return SKKR_OK;
+}
+
 static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, double* tau_bz,
                       double* tr_tau_k, int* iters_k, int* status_k, skkr_stats* st,
                       bool phys, double* tau_bz_phys, double* taukdiag) {
@@ -1075,6 +1106,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
           }
     }
   }
+  if (c->ng_sym > 0) CHK(apply_point_group(c));
   std::vector<double> hbz(2 * tsz);
   HIPCHK(hipMemcpyAsync(hbz.data(), c->taubz, sizeof(z) * tsz, hipMemcpyDeviceToHost, c->stream));
   if (phys) {
@@ -1468,6 +1500,50 @@ int skkr_tau2d_layered_batch(skkr_ctx* c, int nb, const double* m00, const doubl
     if (code != SKKR_OK && rc == SKKR_OK) rc = code;
   }
   return rc;
+}
+
+int skkr_set_point_group(skkr_ctx* c, int ng, const double* dg, const double* dg1) {
+  if (!c || ng < 0 || ng > 48) return SKKR_ERR_ARG;
+  if (ng == 0) {
+    c->ng_sym = 0;
+    return SKKR_OK;
+  }
+  if (!dg || !dg1) return SKKR_ERR_ARG;
+  HIPCHK(hipSetDevice(c->cfg.device));
+  const int lk = c->cfg.lk;
+  const size_t S = (size_t)c->cfg.n_pl * c->ns, bl = (size_t)lk * lk;
+  if (!c->dSymOut) {
+    CHK(dalloc_t(c, &c->dSymOut, S * bl));
+    CHK(dalloc_t(c, &c->dSymTmp, S * bl));
+    std::vector<z*> v(S);
+    for (size_t s = 0; s < S; ++s) v[s] = c->taubz + s * bl;
+    CHK(make_ptrs(c, &c->p_symT, v));
+    for (size_t s = 0; s < S; ++s) v[s] = c->dSymOut + s * bl;
+    CHK(make_ptrs(c, &c->p_symOut, v));
+    for (size_t s = 0; s < S; ++s) v[s] = c->dSymTmp + s * bl;
+    CHK(make_ptrs(c, &c->p_symTmp, v));
+  }
+  if (ng > c->sym_cap) {
+    CHK(dalloc_t(c, &c->dSym, 2 * (size_t)ng * bl));
+    c->p_symD.assign(ng, PtrArr());
+    c->p_symD1.assign(ng, PtrArr());
+    for (int g = 0; g < ng; ++g) {
+      CHK(make_ptrs(c, &c->p_symD[g], std::vector<z*>(S, c->dSym + g * bl)));
+      CHK(make_ptrs(c, &c->p_symD1[g], std::vector<z*>(S, c->dSym + (ng + g) * bl)));
+    }
+    c->sym_cap = ng;
+  } else {
+    // the arrays of a larger earlier group: re-point them for this ng
+    for (int g = 0; g < ng; ++g) {
+      std::vector<z*> d(S, c->dSym + g * bl), d1(S, c->dSym + (ng + g) * bl);
+      HIPCHK(hipMemcpy(c->p_symD[g].d, d.data(), sizeof(z*) * S, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->p_symD1[g].d, d1.data(), sizeof(z*) * S, hipMemcpyHostToDevice));
+    }
+  }
+  HIPCHK(hipMemcpy(c->dSym, dg, sizeof(z) * ng * bl, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dSym + (size_t)ng * bl, dg1, sizeof(z) * ng * bl, hipMemcpyHostToDevice));
+  c->ng_sym = ng;
+  return SKKR_OK;
 }
 
 }  // extern "C"
@@ -2200,6 +2276,7 @@ int skkr_tau_bz_pairs(skkr_ctx* c, double e_re, double e_im, int nk, const doubl
   if (!c || !tau_ij || !tau_ji) return SKKR_ERR_ARG;
   const int np = (int)c->pair_li.size(), lk = c->cfg.lk;
   if (np == 0) return SKKR_ERR_ARG;
+  if (c->ng_sym > 1) return SKKR_ERR_UNSUPPORTED;  // pair sums need the full BZ
   HIPCHK(hipSetDevice(c->cfg.device));
   const size_t ne = (size_t)np * lk * lk;
   HIPCHK(hipMemsetAsync(c->dTij, 0, sizeof(z) * ne, c->stream));

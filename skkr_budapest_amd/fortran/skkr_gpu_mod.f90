@@ -205,5 +205,13 @@ module skkr_gpu
        complex(c_double_complex), intent(out) :: tau(*)
        integer(c_int), intent(out) :: status(*)
      end function skkr_tau2d_layered_batch
+
+     ! point group of an irreducible-wedge mesh (sphbas dg, dg1; cpacoord.f:294-299)
+     integer(c_int) function skkr_set_point_group(ctx, ng, dg, dg1) bind(C, name="skkr_set_point_group")
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: ng
+       complex(c_double_complex), intent(in) :: dg(*), dg1(*)
+     end function skkr_set_point_group
   end interface
 end module skkr_gpu

@@ -35,7 +35,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
            "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs",
-           "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch")
+           "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch", "skkr_set_point_group")
 
 
 class SkkrError(RuntimeError):
@@ -103,6 +103,7 @@ def lib():
         L.skkr_dugo_batch.argtypes = [V, ctypes.c_char, I, V, V, V, V, V, V]
         L.skkr_tau2d_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
         L.skkr_tau2d_layered_batch.argtypes = [V, I, V, V, V, V, V, D, V, V]
+        L.skkr_set_point_group.argtypes = [V, I, V, V]
         L.skkr_set_option.argtypes = [V, ctypes.c_char_p, I]
         L.skkr_autotune.argtypes = [V, ctypes.POINTER(Tuning)]
         L.skkr_set_screening.argtypes = [V, I, V, V]
@@ -378,6 +379,17 @@ class Context:
         status = np.zeros(nb, np.int32)
         lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
         return from_colmajor(tb), status
+
+    def set_point_group(self, dg=None, dg1=None):
+        """Point group of an irreducible-wedge mesh (cpacoord.f:294-299):
+        dg, dg1 (ng, lk, lk); None switches it off."""
+        if dg is None:
+            self._check(lib().skkr_set_point_group(self._h, 0, None, None), "skkr_set_point_group")
+            return
+        d, d1 = colmajor(_c128(dg)), colmajor(_c128(dg1))
+        if d.shape != d1.shape or d.shape[1:] != (self.lk, self.lk):
+            raise ValueError("dg, dg1 must be (ng, lk, lk)")
+        self._check(lib().skkr_set_point_group(self._h, len(d), _p(d), _p(d1)), "skkr_set_point_group")
 
     def tau2d_layered(self, m00, m01, m10, x, y, w=1.0):
         """tau2d of a layered stack (bulkgeo false): m00, m01, m10 are

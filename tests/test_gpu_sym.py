@@ -1,0 +1,99 @@
+"""Point-group symmetrisation of the BZ sum on the GPU (skkr_set_point_group):
+against the reference's symmetrised k-loop (tests/golden/sym_cfg0.npz, C4v
+matrices of the reference's sphbas) and, on the Nb(110) structure constants,
+the identity that fixes it: an irreducible wedge symmetrised with C2v gives
+the full-BZ sum."""
+import os
+
+import numpy as np
+import pytest
+
+from skkr_budapest_amd import kkrsystems as K
+from skkr_budapest_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-11
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+def test_point_group_matches_reference_golden(gpu):
+    g = np.load(os.path.join(GOLD, "sym_cfg0.npz"))
+    sph = np.load(os.path.join(GOLD, "sphbas.npz"))
+    grp = str(g["group"])
+    sy = S.spin_channel(S.make_system(S.CONFIGS[0]), 0)
+    E = complex(g["E"])
+    print("point group", grp, "config 0 spin 1, 256 k", flush=True)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        plain = ctx.tau_bz(E, g["k"], g["w"])
+        ctx.set_point_group(sph[grp + "_dg"], sph[grp + "_dg1"])
+        sym = ctx.tau_bz(E, g["k"], g["w"])
+        ctx.set_point_group(None)
+        again = ctx.tau_bz(E, g["k"], g["w"])
+    assert rel(sym["tau_bz"], g["tau_bz"]) < TOL
+    assert rel(plain["tau_bz"], g["tau_bz"]) > 1e-3      # this structure is not C4v-symmetric
+    assert np.array_equal(sym["trk"], plain["trk"])     # per-k traces are left alone
+    assert np.array_equal(again["tau_bz"], plain["tau_bz"])
+
+
+def _rect_grid(a1, a2, N):
+    """N x N Cartesian grid over the conventional reciprocal rectangle of a
+    centred-rectangular lattice a1 = (a, 0), a2 = (a/2, h): sides 4 pi/a and
+    2 pi/h are reciprocal-lattice vectors, so the grid samples the BZ
+    uniformly; offsets (i + 1/2) make it symmetric in kx and ky."""
+    kx = (np.arange(N) + 0.5) / N * (4 * np.pi / a1[0]) - 2 * np.pi / a1[0]
+    ky = (np.arange(N) + 0.5) / N * (2 * np.pi / a2[1]) - np.pi / a2[1]
+    return np.array([[x, y] for x in kx for y in ky])
+
+
+def test_irreducible_wedge_equals_full_bz_on_nb110(gpu):
+    """The Nb(110) stack (real screened structure constants, lmax 2, t^-1 and
+    screening depending on l only) has the C2v symmetry of its centred-
+    rectangular lattice with mirrors along x and y.  The quarter kx, ky > 0 of
+    a symmetric grid, weight 4/N^2 each, symmetrised with the C2v matrices of
+    the reference's sphbas, must give the full-grid BZ sum."""
+    sph = np.load(os.path.join(GOLD, "sphbas.npz"))
+    E = complex(0.55, 0.05)
+    wl = K.nb110_workload(E, P=2, lmax=2)
+    st_ = wl["structure"]
+    N = 12
+    kf = _rect_grid(st_["a1"], st_["a2"], N)
+    wf = np.full(len(kf), 1.0 / len(kf))
+    q = (kf[:, 0] > 0) & (kf[:, 1] > 0)
+    ki, wi = kf[q], np.full(q.sum(), 4.0 / len(kf))
+    print("irreducible wedge vs full BZ, nb110 lmax 2,", len(kf), "and", len(ki), "k", flush=True)
+    with gpu.Structure(st_["lmax"], st_["s"], st_["nextra"], st_["eta"], st_["a1"], st_["a2"],
+                       st_["cvec"]) as st:
+        with gpu.Context(lk=9, sites_per_pl=st_["s"], n_pl=wl["P"]) as ctx:
+            ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
+            ctx.set_structure_kkr(st, wl["alpha"])
+            full = ctx.tau_bz(E, kf, wf)
+            wedge_plain = ctx.tau_bz(E, ki, wi)
+            ctx.set_point_group(sph["C2v_dg"], sph["C2v_dg1"])
+            wedge = ctx.tau_bz(E, ki, wi)
+            ctx.set_structure_kkr(None, None)
+    assert rel(wedge["tau_bz"], full["tau_bz"]) < 1e-10
+    assert rel(wedge_plain["tau_bz"], full["tau_bz"]) > 1e-4  # the wedge alone is not the BZ sum
+
+
+def test_point_group_arguments(gpu):
+    sy = S.spin_channel(S.make_system(S.CONFIGS[0]), 0)
+    sph = np.load(os.path.join(GOLD, "sphbas.npz"))
+    with gpu.Context.for_system(sy) as ctx:
+        with pytest.raises(ValueError):
+            ctx.set_point_group(sph["C4v_dg"][:, :4, :4], sph["C4v_dg1"][:, :4, :4])
+        ctx.set_point_group(sph["C4v_dg"], sph["C4v_dg1"])
+        ctx.set_pairs([(0, 1, 0, 0)], [1.0, 0.0], [0.0, 1.0])
+        with pytest.raises(gpu.SkkrError):  # pair sums need the full BZ
+            ctx.tau_bz_pairs(0.5 + 0.05j, np.zeros((1, 2)), np.ones(1))

@@ -1,0 +1,52 @@
+"""Point-group symmetrisation of the BZ sum (intbz = 1): the reference's
+sphbas matrices (tests/golden/sphbas.npz) and the reference's symmetrised
+k-loop (tests/golden/sym_cfg0.npz) against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from skkr_budapest_amd import systems as S
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+GROUPS = ("C1", "C2", "C3", "C4", "C6", "Cs", "C2v", "C3vA", "C3vB", "C4v", "C6v")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name", GROUPS)
+def test_sphbas_matrices_form_the_group(name):
+    """What the mathematics fixes: unitary matrices that close under products,
+    dg1 the inverses (also labelled by invg), the identity first, no mixing of
+    different l (lm basis, lmax 2), and the element counts of the groups."""
+    g = np.load(os.path.join(GOLD, "sphbas.npz"))
+    ng, dg, dg1, invg = int(g[name + "_ng"]), g[name + "_dg"], g[name + "_dg1"], g[name + "_invg"]
+    order = {"C1": 1, "C2": 2, "C3": 3, "C4": 4, "C6": 6, "Cs": 2, "C2v": 4, "C3vA": 6, "C3vB": 6,
+             "C4v": 8, "C6v": 12}
+    assert ng == order[name] == len(dg)
+    eye = np.eye(dg.shape[1])
+    lof = np.array([l for l in range(3) for _ in range(2 * l + 1)])
+    assert np.abs(dg[0] - eye).max() < 1e-14
+    for a in range(ng):
+        assert np.abs(dg[a] @ dg[a].conj().T - eye).max() < 1e-14
+        assert np.abs(dg[a] @ dg1[a] - eye).max() < 1e-14
+        assert np.abs(dg1[a] - dg[invg[a] - 1]).max() < 1e-14
+        assert np.abs(dg[a][lof[:, None] != lof[None, :]]).max() == 0.0
+        for b in range(ng):
+            assert min(np.abs(dg[a] @ dg[b] - dg[c]).max() for c in range(ng)) < 1e-14
+
+
+def test_symmetrised_bz_sum_matches_reference():
+    g = np.load(os.path.join(GOLD, "sym_cfg0.npz"))
+    sph = np.load(os.path.join(GOLD, "sphbas.npz"))
+    grp = str(g["group"])
+    sy = S.spin_channel(S.make_system(S.CONFIGS[0]), 0)
+    assert str(g["fingerprint"]) == sy.fingerprint()
+    r = oracle.tau_bz(sy, complex(g["E"]), g["k"], g["w"], nworkers=4)
+    sym = oracle.symmetrize(r["tau_bz"], sph[grp + "_dg"], sph[grp + "_dg1"])
+    assert rel(sym, g["tau_bz"]) < 1e-13
+    # the synthetic structure is not C4v-symmetric: the sum changes
+    assert rel(r["tau_bz"], g["tau_bz"]) > 1e-3
