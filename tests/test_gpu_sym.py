@@ -97,3 +97,30 @@ def test_point_group_arguments(gpu):
         ctx.set_pairs([(0, 1, 0, 0)], [1.0, 0.0], [0.0, 1.0])
         with pytest.raises(gpu.SkkrError):  # pair sums need the full BZ
             ctx.tau_bz_pairs(0.5 + 0.05j, np.zeros((1, 2)), np.ones(1))
+
+
+def test_sample_wedge_with_group_sum_equals_full_bz_mesh(gpu):
+    """The intbz 1 path of the sample end to end: the k-set 21 wedge of the
+    C2vx centred-rectangular Nb(110) lattice (253 k, kmesh2d) with the C2v
+    group sum equals the same k-set unfolded to the full BZ (intbz 2, fullbz),
+    on the Nb(110) structure constants."""
+    from skkr_budapest_amd import kmesh as KM
+    sph = np.load(os.path.join(GOLD, "sphbas.npz"))
+    E = complex(0.55, 0.05)
+    wl = K.nb110_workload(E, P=2, lmax=2)
+    st_ = wl["structure"]
+    lat = KM.primkr2d(2, "C2vx", st_["a1"][0], 0.70710678120, 0.0)
+    kw, ww = KM.kmesh2d(1, 21, lat)
+    kf, wf = KM.kmesh2d(2, 21, lat)
+    print("sample k-set 21: wedge", len(kw), "full", len(kf), flush=True)
+    with gpu.Structure(st_["lmax"], st_["s"], st_["nextra"], st_["eta"], st_["a1"], st_["a2"],
+                       st_["cvec"]) as st:
+        with gpu.Context(lk=9, sites_per_pl=st_["s"], n_pl=wl["P"]) as ctx:
+            ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
+            ctx.set_structure_kkr(st, wl["alpha"])
+            full = ctx.tau_bz(E, kf, wf)
+            ctx.set_point_group(sph["C2v_dg"], sph["C2v_dg1"])
+            wedge = ctx.tau_bz(E, kw, ww)
+            ctx.set_structure_kkr(None, None)
+    assert len(kw) == 253
+    assert rel(wedge["tau_bz"], full["tau_bz"]) < 1e-10
