@@ -10,12 +10,19 @@ c  labels invg) as nrsp-scf/cpacoord.f:98 does.
 c
 c  Output (stream): int32 lmmaxp, melem, ngroups; per group int32 ng,
 c  invg(melem); complex*16 dg(lmmaxp,lmmaxp,melem), dg1(same).
+c  -DREL: rsp-scf/{sphbas,matrep}.f, the kappa-mu basis (double group),
+c  kmymaxp in place of lmmaxp.
       program refsph
       implicit real*8 (a-h,o-z)
       include 'param.h'
       character*4 idgroup
       common/latt2d/idgroup,ilat,a2d,b2d,d2d
-      complex*16 dg(lmmaxp,lmmaxp,melem),dg1(lmmaxp,lmmaxp,melem)
+#ifdef REL
+      parameter(ldg=kmymaxp)
+#else
+      parameter(ldg=lmmaxp)
+#endif
+      complex*16 dg(ldg,ldg,melem),dg1(ldg,ldg,melem)
       integer invg(melem)
       character*4 groups(11)
       character*512 fout
@@ -24,7 +31,7 @@ c  invg(melem); complex*16 dg(lmmaxp,lmmaxp,melem), dg1(same).
       call get_command_argument(1,fout)
       open(12,file=fout,access='stream',form='unformatted',
      &     status='replace')
-      write(12) lmmaxp,melem,11
+      write(12) ldg,melem,11
       do i=1,11
         idgroup=groups(i)
         dg=(0.d0,0.d0)

@@ -3,6 +3,8 @@
 sphbas.npz : the matrices of every 2D point group of the reference's
     sphbas (skkr_kpar_v6/nrsp-scf/sphbas.f with matrep.f; lm basis, lmax 2):
     ng, invg, dg, dg1 per group (oracle/_ref/refsph).
+sphbas_rel.npz : the same for the kappa-mu basis (rsp-scf/sphbas.f with
+    matrep.f; double group, 18 x 18 for lmax 2; oracle/_ref/refsph_rel).
 sym_cfg0.npz : the BZ sum of config 0's first spin channel (lk 9, 2 sites,
     8 PLs, square lattice, E = 0.5+0.05i, 16x16 mesh) symmetrised with the
     C4v matrices as nrsp-scf/cpacoord.f:281-299 does per k:
@@ -32,10 +34,10 @@ REFDIR = os.path.join(ROOT, "oracle", "_ref")
 GROUPS = ("C1", "C2", "C3", "C4", "C6", "Cs", "C2v", "C3vA", "C3vB", "C4v", "C6v")
 
 
-def run_sphbas():
+def run_sphbas(exe="refsph"):
     with tempfile.TemporaryDirectory() as td:
         fout = os.path.join(td, "sph.bin")
-        subprocess.run([os.path.join(REFDIR, "refsph"), fout], capture_output=True, check=True)
+        subprocess.run([os.path.join(REFDIR, exe), fout], capture_output=True, check=True)
         raw = open(fout, "rb").read()
     o = 0
 
@@ -92,6 +94,7 @@ def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "ref")], check=True)
     sph = run_sphbas()
     np.savez_compressed(os.path.join(HERE, "sphbas.npz"), **sph)
+    np.savez_compressed(os.path.join(HERE, "sphbas_rel.npz"), **run_sphbas("refsph_rel"))
     sy = S.spin_channel(S.make_system(S.CONFIGS[0]), 0)
     out = run_sym(sy, S.CONFIGS[0].energies[0], 16, sph["C4v_dg"], sph["C4v_dg1"])
     np.savez_compressed(os.path.join(HERE, "sym_cfg0.npz"), group=np.array("C4v"), **out)

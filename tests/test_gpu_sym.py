@@ -124,3 +124,28 @@ def test_sample_wedge_with_group_sum_equals_full_bz_mesh(gpu):
             ctx.set_structure_kkr(None, None)
     assert len(kw) == 253
     assert rel(wedge["tau_bz"], full["tau_bz"]) < 1e-10
+
+
+def test_relativistic_wedge_with_group_sum_equals_full_bz(gpu):
+    """The kappa-mu k-loop on the producer (relmtrx basis of clebsch.f) with
+    the double-group C2v matrices of the reference's rsp-scf/sphbas.f: the
+    k-set 9 wedge with the group sum equals its full-BZ unfolding."""
+    from skkr_budapest_amd import kmesh as KM
+    sph = np.load(os.path.join(GOLD, "sphbas_rel.npz"))
+    E = complex(0.55, 0.05)
+    wl = K.nb110_workload(E, P=2, lmax=2, rel=1)
+    st_ = wl["structure"]
+    lat = KM.primkr2d(2, "C2vx", st_["a1"][0], 0.70710678120, 0.0)
+    kw, ww = KM.kmesh2d(1, 9, lat)
+    kf, wf = KM.kmesh2d(2, 9, lat)
+    print("relativistic wedge", len(kw), "full", len(kf), flush=True)
+    with gpu.Structure(st_["lmax"], st_["s"], st_["nextra"], st_["eta"], st_["a1"], st_["a2"],
+                       st_["cvec"]) as st:
+        with gpu.Context(lk=18, sites_per_pl=st_["s"], n_pl=wl["P"]) as ctx:
+            ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
+            ctx.set_structure_kkr(st, wl["alpha"], rel=1)
+            full = ctx.tau_bz(E, kf, wf)
+            ctx.set_point_group(sph["C2v_dg"], sph["C2v_dg1"])
+            wedge = ctx.tau_bz(E, kw, ww)
+            ctx.set_structure_kkr(None, None)
+    assert rel(wedge["tau_bz"], full["tau_bz"]) < 1e-10

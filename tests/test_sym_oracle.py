@@ -17,26 +17,36 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("basis", ["lm", "kappa-mu"])
 @pytest.mark.parametrize("name", GROUPS)
-def test_sphbas_matrices_form_the_group(name):
-    """What the mathematics fixes: unitary matrices that close under products,
-    dg1 the inverses (also labelled by invg), the identity first, no mixing of
-    different l (lm basis, lmax 2), and the element counts of the groups."""
-    g = np.load(os.path.join(GOLD, "sphbas.npz"))
+def test_sphbas_matrices_form_the_group(name, basis):
+    """What the mathematics fixes: unitary matrices that close under products
+    (up to sign for the double group of the kappa-mu basis), dg1 the inverses
+    (also labelled by invg), the identity first, no mixing of different l
+    (lmax 2), and the element counts of the groups."""
+    rel = basis == "kappa-mu"
+    g = np.load(os.path.join(GOLD, "sphbas_rel.npz" if rel else "sphbas.npz"))
     ng, dg, dg1, invg = int(g[name + "_ng"]), g[name + "_dg"], g[name + "_dg1"], g[name + "_invg"]
     order = {"C1": 1, "C2": 2, "C3": 3, "C4": 4, "C6": 6, "Cs": 2, "C2v": 4, "C3vA": 6, "C3vB": 6,
              "C4v": 8, "C6v": 12}
     assert ng == order[name] == len(dg)
     eye = np.eye(dg.shape[1])
-    lof = np.array([l for l in range(3) for _ in range(2 * l + 1)])
+    if rel:
+        from skkr_budapest_amd.kkrsystems import clebsch_kappa_l
+        lof = np.array(clebsch_kappa_l(2))
+    else:
+        lof = np.array([l for l in range(3) for _ in range(2 * l + 1)])
+    assert len(lof) == dg.shape[1]
     assert np.abs(dg[0] - eye).max() < 1e-14
     for a in range(ng):
         assert np.abs(dg[a] @ dg[a].conj().T - eye).max() < 1e-14
         assert np.abs(dg[a] @ dg1[a] - eye).max() < 1e-14
-        assert np.abs(dg1[a] - dg[invg[a] - 1]).max() < 1e-14
+        lab = dg[invg[a] - 1]
+        assert min(np.abs(dg1[a] - lab).max(), np.abs(dg1[a] + lab).max() if rel else 1.0) < 1e-14
         assert np.abs(dg[a][lof[:, None] != lof[None, :]]).max() == 0.0
         for b in range(ng):
-            assert min(np.abs(dg[a] @ dg[b] - dg[c]).max() for c in range(ng)) < 1e-14
+            signs = (1, -1) if rel else (1,)
+            assert min(np.abs(dg[a] @ dg[b] - sg * dg[c]).max() for c in range(ng) for sg in signs) < 1e-14
 
 
 def test_symmetrised_bz_sum_matches_reference():
