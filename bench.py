@@ -206,6 +206,73 @@ def kkr_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def kkr_sample_bench(args):
+    """The energy loop of the reference's sample 02-layer-scf on the layer
+    producer: the Nb(110) C2vx stack (input_geo.in: NPRC 3 x 4 layers, a =
+    4.667470441, beta = 0.7071), lmax 2 relativistic (kappa-mu, lk 18), the
+    16 contour energies of input_rsp.in:32 (zmesh) with the k-sets of
+    input_rsp.in:35-39 (intbz 1 wedges of kmesh2d: 253 ... 1953 k) and the C2v
+    group sum of the reference's rsp sphbas.  t^-1 and screening are square
+    wells (the sample's potentials are not used)."""
+    from skkr_budapest_amd import contour, gpu
+    from skkr_budapest_amd import kkrsystems as K
+    from skkr_budapest_amd import kmesh as KM
+    z, _ = contour.semicircle(16, -1.0, -0.0183132794260, 150.0)
+    ksets = [21] * 10 + [31] * 2 + [41] * 2 + [51] + [61]
+    st = K.nb110_workload(z[0], P=3, lmax=2, rel=1)["structure"]
+    lat = KM.primkr2d(2, "C2vx", st["a1"][0], 0.70710678120, 0.0)
+    meshes = {s: KM.kmesh2d(1, s, lat) for s in sorted(set(ksets))}
+    sph = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "sphbas_rel.npz"))
+    nk_total = sum(len(meshes[s][1]) for s in ksets)
+    print(f"bench: sample 02-layer-scf energy loop, 16 energies, {nk_total} k-evaluations", file=sys.stderr,
+          flush=True)
+
+    def one_energy(ctx, sc, ie):
+        E = complex(z[ie])
+        wl = K.nb110_workload(E, P=3, lmax=2, rel=1)
+        ctx.set_tinv(wl["tinv"], wl["tinv_lead"], wl["tinv_lead"])
+        ctx.set_structure_kkr(sc, wl["alpha"], rel=1)
+        k, w = meshes[ksets[ie]]
+        t0 = time.perf_counter()
+        r = ctx.tau_bz(E, k, w, raise_on_error=False)
+        if r["rc"] not in (0, 6):  # 6: decimation reached itermax somewhere (surfgfd warns and goes on)
+            raise gpu.SkkrError(r["rc"], f"tau_bz at E={E}")
+        return time.perf_counter() - t0, r
+
+    with gpu.Structure(**st) as sc, gpu.Context(lk=18, sites_per_pl=st["s"], n_pl=3) as ctx:
+        ctx.set_point_group(sph["C2v_dg"], sph["C2v_dg1"])
+        for _ in range(args.warmup):
+            one_energy(ctx, sc, 0)
+            one_energy(ctx, sc, 15)
+        loops, per_e = [], []
+        for step in range(args.steps):
+            tot, rows = 0.0, []
+            for ie in range(16):
+                dt, r = one_energy(ctx, sc, ie)
+                tot += dt
+                rows.append({"E": [z[ie].real, z[ie].imag], "nk": len(meshes[ksets[ie]][1]),
+                             "ms": 1e3 * dt, "mean_iter": 0.5 * float(np.mean(r["iters"])),
+                             "max_iter": int(r["iters"].max()), "k_at_itermax": int(np.sum(r["status"] != 0))})
+            loops.append(tot)
+            per_e = rows
+            print(f"bench: contour {step}: {tot:.3f} s", file=sys.stderr, flush=True)
+        ctx.set_structure_kkr(None, None)
+    sec = float(np.mean(loops))
+    out = {
+        "metric": METRIC, "value": nk_total / sec, "unit": "k-points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "sample 02-layer-scf geometry, contour and k-sets; square-well t^-1 and screening",
+        "config": {"workload": "sample 02-layer-scf energy loop: Nb(110) C2vx, 3 PLs x 4 layers, lmax 2 "
+                               "kappa-mu (lk 18, n 72), 16 contour energies, intbz 1 wedges with the C2v "
+                               "group sum",
+                   "precision": "fp64", "k_evaluations": nk_total, "per_energy": per_e},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +282,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kkr", action="store_true",
                     help="k-loop on the layer structure-constant producer (Nb(110) stack)")
+    ap.add_argument("--kkr-sample", action="store_true",
+                    help="the energy loop of the reference's sample 02-layer-scf on the layer producer")
     ap.add_argument("--kkr-rel", action="store_true",
                     help="with --kkr: the relativistic lmax-3 stack of config 1's shape "
                          "(lk 32, 4 sites/PL, 20 PLs) on the producer")
@@ -228,6 +297,8 @@ def main():
     from skkr_budapest_amd import gpu
     from skkr_budapest_amd import systems as S
 
+    if args.kkr_sample:
+        return kkr_sample_bench(args)
     if args.kkr:
         return kkr_bench(args)
     cfg = S.CONFIGS[args.config]
