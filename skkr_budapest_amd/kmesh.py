@@ -6,12 +6,11 @@ orientation and the boundary weights, with the lattice set-up these rely on
 nrsp-scf/readgeom.f:51).  The loops and the order of operations are the
 reference's, so the point sets and weights are the same numbers.
 
-Covered: equidistant points (kunning = 0) on all five 2D lattices; intbz 0/1
-(irreducible wedge) for the lattice's full point group, where ibz.f keeps
-every wedge point once, and for a single k (iset = 0); intbz 2 (full BZ)
-for every group.  Not covered: Cunningham points (kunning = 1, speck.f) and
-wedges of lower-symmetry groups (the per-group skip tables of ibz.f), which
-raise NotImplementedError.
+Covered: equidistant points (kunning = 0, speck2.f) on all five 2D lattices
+and Cunningham special points (kunning = 1, setspk.f and speck.f; the
+oblique lattice falls back to equidistant points as kmesh2d.f does); intbz
+0/1 (the irreducible BZ of any allowed group, ibz.f with its skip tables)
+and intbz 2 (full BZ); single k (iset = 0).
 """
 from __future__ import annotations
 
@@ -84,8 +83,14 @@ def primkr2d(ilat, idgroup, a2d, b2d=0.0, d2d=0.0):
     dnt2 = ay[0] * v2[0] + ay[1] * v2[1]
     bx = (2.0 * pi * v1[0] / dnt1, 2.0 * pi * v1[1] / dnt1)
     by = (2.0 * pi * v2[0] / dnt2, 2.0 * pi * v2[1] / dnt2)
+    v1 = (ayc[1], -ayc[0])
+    v2 = (-axc[1], axc[0])
+    dnt1 = axc[0] * v1[0] + axc[1] * v1[1]
+    dnt2 = ayc[0] * v2[0] + ayc[1] * v2[1]
+    bxc = (2.0 * pi * v1[0] / dnt1, 2.0 * pi * v1[1] / dnt1)
+    byc = (2.0 * pi * v2[0] / dnt2, 2.0 * pi * v2[1] / dnt2)
     return dict(ilat=ilat, a2d=a2d, beta=b2d, delta=d2d, ax=ax, ay=ay, bx=bx, by=by, phik=arot,
-                idgroup=idgroup, iogroup=iogroup, axc=axc, ayc=ayc)
+                idgroup=idgroup, iogroup=iogroup, axc=axc, ayc=ayc, bxc=bxc, byc=byc)
 
 
 def rgroup(idgroup):
@@ -223,6 +228,236 @@ def speck2(n, ilat, alat, beta, delta):
     return [const * x for x in xk], [const * y for y in yk], wk
 
 
+# setspk.f: the requested number of Cunningham points per k-set and lattice,
+# and the largest k-set each lattice takes (iset is clamped to it)
+_SETSPK = {1: (8, {1: 1, 2: 2, 3: 4, 4: 8, 5: 16, 6: 32, 7: 64, 8: 128}),
+           2: (4, {1: 1, 2: 4, 3: 16, 4: 64}),
+           3: (4, {1: 1, 2: 4, 3: 16, 4: 64}),
+           4: (5, {1: 1, 2: 3, 3: 10, 4: 36, 5: 136}),
+           5: (6, {1: 1, 2: 3, 3: 6, 4: 18, 5: 45, 6: 135})}
+# log(2.0) and log(4.0) of speck.f are single-precision constants
+_LOG2_F32 = float(np.float32(math.log(2.0)))
+_LOG4_F32 = float(np.float32(math.log(4.0)))
+
+
+def setspk(ilat, iset):
+    """main/setspk.f: (clamped iset, requested number of special points)."""
+    top, tab = _SETSPK[ilat]
+    iset = min(iset, top)
+    return iset, tab[iset]
+
+
+def speck(nspk, lat):
+    """main/speck.f:5-254: Cunningham special points of the irreducible wedge
+    of the centred and primitive rectangular, square and hexagonal lattices
+    (the smallest set with at least nspk points), in the unrotated reciprocal
+    basis bxc, byc of primkr2d.  kmesh2d never asks for the oblique lattice."""
+    stol = tol = 1.0e-6
+    ilat = lat["ilat"]
+    rlvax, rlvay = lat["bxc"]
+    rlvby = lat["byc"][1]
+    sr3 = math.sqrt(3.0)
+    if ilat in (2, 3):
+        xkg, ykg = rlvax / 2.0, rlvby / 8.0
+        if ilat == 3:
+            xkg, ykg = xkg / 2.0, ykg * 2.0
+        px, py = [xkg], [ykg]
+        if nspk == 1:
+            return px, py
+        nt = int(math.log(float(nspk)) / _LOG4_F32 + 1.0 - stol)
+        nspk = 4 ** (nt + 1) // 4
+        while True:
+            xkg, ykg = xkg / 2.0, ykg / 2.0
+            nx, ny = [], []
+            for ox, oy in zip(px, py):
+                x1, y1, x2, y2 = ox + xkg, oy + ykg, ox - xkg, oy - ykg
+                nx += [x1, x1, x2, x2]
+                ny += [y1, y2, y1, y2]
+            px, py = nx, ny
+            if len(px) >= nspk:
+                break
+        if ilat == 2:
+            beta = -rlvax / rlvay
+            for i in range(len(px)):
+                t = py[i] + 0.5 * px[i] / beta - 0.75 * rlvax / beta
+                if t > 0.0:
+                    px[i], py[i] = -px[i] + rlvax, -py[i] - rlvay
+        return px, py
+    if ilat == 4:
+        xkg = ykg = rlvax / 4.0
+        px, py = [xkg], [ykg]
+        if nspk <= 1:
+            return px, py
+        nt = int(math.log((-1.0 + math.sqrt(1.0 + 8.0 * float(nspk))) / 2.0) / _LOG2_F32 + 1.0 - stol)
+        nu = 2 ** (nt + 1) // 2
+        nspk = (nu * nu + nu) // 2
+        insp = 0
+        while True:
+            nt = 2 ** (insp + 1) // 2
+            if (nt * nt + nt) // 2 >= nspk:
+                return px, py
+            xkg, ykg = xkg / 2.0, ykg / 2.0
+            nx, ny = [], []
+            for ox, oy in zip(px, py):
+                xs, ys = (ox + xkg, ox - xkg), (oy + ykg, oy - ykg)
+                for xv in xs:
+                    for yv in ys:
+                        if xv < yv:
+                            continue
+                        nx.append(xv)
+                        ny.append(yv)
+            px, py = nx, ny
+            insp += 1
+    if ilat == 5:
+        hx = (1.0, 0.5, -0.5, -1.0, -0.5, 0.5)
+        hy = (0.0, 0.5 * sr3, 0.5 * sr3, 0.0, -0.5 * sr3, -0.5 * sr3)
+        sa = 0.5 * rlvax
+        xs = [4.0 * sa / 3.0, 2.0 * sa / 3.0]
+        ys = [0.0, 2.0 * sa / 3.0 / sr3]
+        px, py = [xs[1]], [ys[1]]
+        if nspk <= 1:
+            return px, py
+        ix = 2
+        while True:
+            ix = 2 - (ix + 1) % 2
+            xs[ix - 1] /= 3.0
+            ys[ix - 1] /= 3.0
+            xkg, ykg = xs[ix - 1], ys[ix - 1]
+            nx, ny = [], []
+            for xa, ya in zip(px, py):
+                for j in range(1, 7):
+                    jm = j % 6
+                    xap = hx[jm] * xkg + hy[jm] * ykg + xa
+                    yap = -hy[jm] * xkg + hx[jm] * ykg + ya
+                    if yap < -tol or (yap - xap / sr3) > tol or (yap + xap * sr3 - 2.0 / sr3 * rlvax) > tol:
+                        continue
+                    if any(abs(xap - qx) < tol and abs(yap - qy) < tol for qx, qy in zip(nx, ny)):
+                        continue
+                    nx.append(xap)
+                    ny.append(yap)
+            px, py = nx, ny
+            if len(px) >= nspk:
+                return px, py
+    raise ValueError("speck: lattice type must be 2..5")
+
+
+def _ibz(lat, kunning, xk1, yk1):
+    """main/ibz.f: the wedge of the lattice's holohedral group (xk1, yk1)
+    unfolded into the irreducible BZ of the input group, with the skip
+    tables that keep every edge point once."""
+    tol = 1.0e-8
+    pi = 4.0 * math.atan(1.0)
+    ilat, idgroup, iogroup = lat["ilat"], lat["idgroup"], lat["iogroup"]
+    amat = rgroup(HOLOHEDRY[ilat])
+    ng = len(amat)
+    xo, yo = [], []
+    for x, y in zip(xk1, yk1):
+        ngeff = ng
+        skip = set()
+        if ilat == 1:
+            if idgroup == "C2":
+                ngeff = 1
+            if (abs(x - 0.0) < tol) if kunning == 0 else (abs(y - 0.0) < tol):
+                ngeff = 1
+        elif ilat in (2, 3):
+            if idgroup == "C2v":
+                ngeff = 1
+            elif idgroup in ("Cs", "C2"):
+                if iogroup == "Csy":
+                    if abs(y - 0.0) < tol:
+                        ngeff = 1
+                    else:
+                        skip, ngeff = {2}, 3
+                elif abs(x - 0.0) < tol:
+                    ngeff = 1
+                else:
+                    skip = {2, 3}
+            elif abs(x - 0.0) < tol or abs(y - 0.0) < tol:
+                ngeff = 2
+        elif ilat == 4:
+            if idgroup == "C4v":
+                ngeff = 1
+            elif idgroup in ("C2v", "C4"):
+                if iogroup == "C2vd":
+                    ngeff, skip = 6, {2, 3, 4, 5}
+                    if abs(y - 0.0) < tol:
+                        ngeff = 1
+                elif abs(x - y) < tol:
+                    ngeff = 1
+                else:
+                    skip = set(range(2, ng))
+            elif idgroup in ("C2", "Cs"):
+                skip = {3, 4, 7}
+                if iogroup in ("Csd1", "Csd2"):
+                    skip.add(5)
+                    if abs(x - y) < tol:
+                        ngeff = 6
+                    elif abs(y - 0.0) < tol:
+                        ngeff = 2
+                else:
+                    skip.add(6)
+                    if abs(x - y) < tol:
+                        ngeff = 2
+                    elif abs(y - 0.0) < tol:
+                        ngeff = 5
+            elif abs(x - y) < tol or abs(y - 0.0) < tol:
+                ngeff = 4
+        else:
+            phi = _asin_phi(x, y)
+            at0, at30 = abs(phi - 0.0) < tol, abs(phi - pi / 6.0) < tol
+            if idgroup == "C6v":
+                ngeff = 1
+            elif idgroup in ("C6", "C3vB"):
+                if at0:
+                    ngeff = 1
+                else:
+                    ngeff, skip = 10, set(range(2, 10))
+            elif idgroup == "C3vA":
+                if at30:
+                    ngeff = 1
+                else:
+                    ngeff, skip = 8, set(range(2, 8))
+            elif idgroup == "C3":
+                if at30:
+                    ngeff = 2
+                else:
+                    skip = set(range(3, ngeff)) - {8}
+                    if at0:
+                        skip.add(8)
+            elif idgroup == "C2v":
+                if iogroup in ("C2v2", "C2v3"):
+                    ngeff, skip = 10, set(range(2, 8)) | {9}
+                    if at30:
+                        skip.add(8)
+                    if at0:
+                        ngeff = 8
+                elif at30 or at0:
+                    ngeff = 2
+                else:
+                    ngeff, skip = 8, set(range(3, 8))
+            elif idgroup in ("Cs", "C2"):
+                if iogroup in ("Csy", "Csy2", "Csy3"):
+                    skip, ngeff = set(range(4, 8)), 10
+                    if at30:
+                        skip |= {8, 10}
+                    if at0:
+                        ngeff = 3
+                else:
+                    skip = set(range(3, ngeff)) - {4, 7, 8}
+                    if at30:
+                        ngeff = 4
+                    if at0:
+                        ngeff = 7
+            elif at30 or at0:
+                ngeff = 6
+        for ig in range(1, ngeff + 1):
+            if ig not in skip:
+                a = amat[ig - 1]
+                xo.append(a[0, 0] * x + a[0, 1] * y)
+                yo.append(a[1, 0] * x + a[1, 1] * y)
+    return xo, yo
+
+
 def _findangles_phi(x, y):
     """main/findangles.f with z = 0: the azimuth in [0, 2 pi)."""
     tol = 1.0e-5
@@ -303,16 +538,16 @@ def kmesh2d(intbz, iset, lat, park=(0.0, 0.0), kunning=0):
     else:
         if ilat == 1 and kunning == 1:
             kunning = 0   # kmesh2d.f: no Cunningham points on the oblique lattice
-        if kunning != 0:
-            raise NotImplementedError("Cunningham points (kunning = 1, speck.f) are not restated")
-        xk0, yk0, _ = speck2(iset, ilat, lat["a2d"], lat["beta"], lat["delta"])
+        if kunning == 0:
+            xk0, yk0, _ = speck2(iset, ilat, lat["a2d"], lat["beta"], lat["delta"])
+        else:
+            iset, nspk = setspk(ilat, iset)
+            xk0, yk0 = speck(nspk, lat)
     if intbz <= 1:
         if iset == 0:
             xk, yk = list(xk0), list(yk0)
         else:
-            if idgroup != HOLOHEDRY[ilat]:
-                raise NotImplementedError("wedges of groups below the lattice's full symmetry (ibz.f)")
-            xk, yk = list(xk0), list(yk0)   # ibz.f: ngeff = 1, the identity
+            xk, yk = _ibz(lat, kunning, xk0, yk0)
     else:
         xk, yk = _fullbz(lat, iset, kunning, xk0, yk0)
     phi = phik

@@ -1,6 +1,8 @@
 """The 2D k-meshes (skkr_budapest_amd.kmesh, a restatement of kmesh2d.f)
-against the reference's own primkr2d/rgroup/kmesh2d (tests/golden/kmesh.npz)
-and the k-counts the sample prints."""
+against the reference's own primkr2d/rgroup/kmesh2d (tests/golden/kmesh.npz:
+equidistant and Cunningham sets, every group's irreducible BZ, full BZ), the
+k-counts the sample prints, and the identity that fixes an irreducible BZ:
+its group sum is the full-BZ sum."""
 import os
 import sys
 
@@ -17,8 +19,8 @@ from make_kmesh_golden import A_NB, CASES  # noqa: E402
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_kmesh_matches_reference(case):
     g = np.load(os.path.join(GOLD, "kmesh.npz"))
-    name, ilat, grp, a, b, d, intbz, iset, park = case
-    k, w = KM.kmesh2d(intbz, iset, KM.primkr2d(ilat, grp, a, b, d), park)
+    name, ilat, grp, a, b, d, intbz, iset, park, *kun = case
+    k, w = KM.kmesh2d(intbz, iset, KM.primkr2d(ilat, grp, a, b, d), park, kunning=kun[0] if kun else 0)
     kr, wr = g[name + "_k"], g[name + "_w"]
     assert k.shape == kr.shape
     assert np.abs(k - kr).max() <= 2e-15 * max(1.0, np.abs(kr).max())
@@ -36,10 +38,59 @@ def test_sample_k_sets_have_the_printed_sizes():
         assert len(w) == nk
 
 
-def test_unrestated_settings_raise():
-    with pytest.raises(NotImplementedError):   # wedge of a subgroup (ibz.f tables)
-        KM.kmesh2d(1, 8, KM.primkr2d(4, "C2vx", 1.0))
-    with pytest.raises(NotImplementedError):   # Cunningham points
-        KM.kmesh2d(1, 8, KM.primkr2d(4, "C4v", 1.0), kunning=1)
+def test_invalid_lattice_group_raises():
     with pytest.raises(ValueError):
         KM.primkr2d(2, "C4v", 1.0, 0.7)
+
+
+def _fourier(k, lat, c, ns):
+    """A lattice-periodic function: sum_R c_R exp(i k.R) over lattice vectors
+    R = n1 ax + n2 ay of the (rotated) lattice."""
+    ax, ay = np.array(lat["ax"]), np.array(lat["ay"])
+    R = np.array([n1 * ax + n2 * ay for n1, n2 in ns])
+    return (np.exp(1j * k @ R.T) * c).sum(-1)
+
+
+IBZ_CASES = [c for c in CASES if c[6] == 1 and c[7] != 0 and not (c[1] == 5 and c[2] == "C6v")]
+
+
+@pytest.mark.parametrize("case", IBZ_CASES, ids=[c[0] for c in IBZ_CASES])
+def test_irreducible_bz_with_its_group_is_the_full_bz(case):
+    """What fixes an irreducible-BZ mesh of a group G: for every lattice-
+    periodic f, sum_k w_k (1/|G|) sum_g f(g k) over the irreducible BZ equals
+    the full-BZ mesh sum (intbz 2) of the same k-set.  A dropped or doubled
+    edge point, a wrong skip table or edge weight, or a wrong rotation breaks
+    it.  G is rgroup's, except for Cs on the rectangular lattices: rgroup.f
+    calls diag(1,-1) "Sx" in Cs and diag(-1,1) "Sx" in C2v, and ibz.f builds
+    the rectangular Cs wedge from C2v's element 4 (diag(1,-1)), so that mesh
+    is the irreducible BZ of the mirror x -> -x."""
+    name, ilat, grp, a, b, d, intbz, iset, park, *kun = case
+    kun = kun[0] if kun else 0
+    lat = KM.primkr2d(ilat, grp, a, b, d)
+    k, w = KM.kmesh2d(1, iset, lat, kunning=kun)
+    kf, wf = KM.kmesh2d(2, iset, lat, kunning=kun)
+    G = KM.rgroup(lat["idgroup"])
+    if ilat in (2, 3) and lat["idgroup"] == "Cs":
+        G = [np.eye(2), np.diag([-1.0, 1.0])]
+    ns = [(n1, n2) for n1 in range(-2, 3) for n2 in range(-2, 3)]
+    rng = np.random.default_rng(len(name))
+    c = rng.standard_normal(len(ns)) + 1j * rng.standard_normal(len(ns))
+    lhs = sum((w * _fourier(k @ g.T, lat, c, ns)).sum() for g in G) / len(G)
+    rhs = (wf * _fourier(kf, lat, c, ns)).sum()
+    assert abs(lhs - rhs) < 1e-13
+
+
+def test_hexagonal_c6v_wedge_keeps_full_weight_on_its_mirror_line():
+    """The reference's C6v wedge of the hexagonal lattice (speck2.f) lies
+    between phi = 0 and phi = 30 degrees, while kmesh2d.f halves the weights
+    on x = 0 and phi = 30 degrees: the wedge points on the phi = 0 mirror
+    keep full weight though they have 6 images instead of 12.  The
+    restatement keeps this (the point sets and weights above are the
+    reference's); the symmetrised wedge then differs from the full-BZ sum by
+    O(1/N), which goes to zero with the k-set."""
+    lat = KM.primkr2d(5, "C6v", 1.0)
+    k, w = KM.kmesh2d(1, 9, lat)
+    on_mirror = (np.abs(k[:, 1]) < 1e-12) & (np.abs(k[:, 0]) > 1e-12)
+    inner = ~on_mirror & (w > 0.9 * w.max())
+    assert on_mirror.sum() > 0
+    assert np.allclose(w[on_mirror], w[inner].max())
