@@ -410,31 +410,32 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
   return SKKR_OK;
 }
 
+// The blocked kernel for T = ceil(n / 16) block columns (the matrix padded
+// with the identity to 16 T rows): T x T tiles of 16 x 16 over
+// (T / TI) x (T / TJ) waves, each holding TI x TJ tiles in registers.  The
+// work goes as T^3, so n = 72 runs as 80 rather than 128 rows.
+template <bool PIV>
+static int launch_gjinv_blk(const InvArgs& a, int nb, hipStream_t st) {
+  switch ((a.n + 15) / 16) {
+    case 1:
+    case 2: gjinv_blk_f64_kernel<2, 2, 2, PIV><<<nb, 64, 0, st>>>(a); break;
+    case 3:  // a T = 3 instance spills to scratch
+    case 4: gjinv_blk_f64_kernel<4, 2, 2, PIV><<<nb, 256, 0, st>>>(a); break;
+    case 5: gjinv_blk_f64_kernel<5, 1, 5, PIV><<<nb, 320, 0, st>>>(a); break;
+    case 6: gjinv_blk_f64_kernel<6, 2, 3, PIV><<<nb, 384, 0, st>>>(a); break;
+    case 7: gjinv_blk_f64_kernel<7, 1, 7, PIV><<<nb, 448, 0, st>>>(a); break;
+    case 8: gjinv_blk_f64_kernel<8, 2, 4, PIV><<<nb, 512, 0, st>>>(a); break;
+    default: return SKKR_ERR_UNSUPPORTED;
+  }
+  HIPCHK(hipGetLastError());
+  return SKKR_OK;
+}
+
 static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
   if (nb <= 0) return SKKR_OK;
-  if (c->inv_kernel == 0) {  // blocked MFMA Gauss-Jordan
-    if (a.n <= 32)
-      gjinv_blk_f64_kernel<2, 2, 2><<<nb, 64, 0, c->stream>>>(a);
-    else if (a.n <= 64)
-      gjinv_blk_f64_kernel<4, 2, 2><<<nb, 256, 0, c->stream>>>(a);
-    else if (a.n <= 128)
-      gjinv_blk_f64_kernel<8, 2, 4><<<nb, 512, 0, c->stream>>>(a);
-    else
-      return SKKR_ERR_UNSUPPORTED;
-    HIPCHK(hipGetLastError());
-    c->launches++;
-    return SKKR_OK;
-  }
-  if (c->inv_kernel == 2) {  // blocked, no pivoting inside the diagonal blocks
-    if (a.n <= 32)
-      gjinv_blk_f64_kernel<2, 2, 2, false><<<nb, 64, 0, c->stream>>>(a);
-    else if (a.n <= 64)
-      gjinv_blk_f64_kernel<4, 2, 2, false><<<nb, 256, 0, c->stream>>>(a);
-    else if (a.n <= 128)
-      gjinv_blk_f64_kernel<8, 2, 4, false><<<nb, 512, 0, c->stream>>>(a);
-    else
-      return SKKR_ERR_UNSUPPORTED;
-    HIPCHK(hipGetLastError());
+  if (a.n < 1) return SKKR_ERR_ARG;
+  if (c->inv_kernel == 0 || c->inv_kernel == 2) {  // blocked MFMA Gauss-Jordan (2: no pivoting in the 16 x 16 blocks)
+    CHK(c->inv_kernel == 0 ? launch_gjinv_blk<true>(a, nb, c->stream) : launch_gjinv_blk<false>(a, nb, c->stream));
     c->launches++;
     return SKKR_OK;
   }

@@ -66,7 +66,7 @@ def test_zgemm_three_product_matches_numpy(gpu, n):
 
 
 @pytest.mark.parametrize("kernel", ["blocked", "rowwise"])
-@pytest.mark.parametrize("n", [1, 9, 18, 36, 64, 100, 128])
+@pytest.mark.parametrize("n", [1, 9, 18, 36, 48, 64, 65, 72, 80, 90, 96, 100, 112, 128])
 def test_gjinv_matches_numpy(gpu, n, kernel, monkeypatch):
     monkeypatch.setenv("SKKR_INV_KERNEL", kernel)
     rng = np.random.default_rng(100 + n)
@@ -81,7 +81,7 @@ def test_gjinv_matches_numpy(gpu, n, kernel, monkeypatch):
         assert (st == 4).all()
 
 
-@pytest.mark.parametrize("n", [9, 36, 64, 128])
+@pytest.mark.parametrize("n", [9, 36, 64, 72, 96, 112, 128])
 def test_gjinv_without_diagonal_pivoting_on_accretive_matrices(gpu, n):
     """inv_kernel 2 (no row exchanges inside the 16x16 diagonal blocks) on
     matrices of the layer problem's class: E - H with Im E > 0, whose
