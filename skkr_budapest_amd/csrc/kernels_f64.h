@@ -295,11 +295,18 @@ __global__ __launch_bounds__((TM / 32) * (TN / (16 * WT)) * 64) void zgemm_f64_k
         }
       }
   if (epi.mode == EPI_DEC_ALPHA) {
-    // deterministic work-group sum of |alpha'|^2 (tree in LDS, fixed order)
+    // deterministic work-group sum of |alpha'|^2 (tree in LDS, fixed order);
+    // a tail above the largest power of two P <= NT (9 waves) is folded first
+    constexpr int P = (NT >= 512) ? 512 : (NT >= 256) ? 256 : (NT >= 128) ? 128 : 64;
+    static_assert(NT >= 64 && NT - P <= P, "work-group size");
     __shared__ double red[NT];
     red[tid] = part;
     __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
+    if (P < NT) {
+      if (tid < NT - P) red[tid] += red[tid + P];
+      __syncthreads();
+    }
+    for (int st = P / 2; st > 0; st >>= 1) {
       if (tid < st) red[tid] += red[tid + st];
       __syncthreads();
     }

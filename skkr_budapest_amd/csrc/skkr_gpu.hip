@@ -117,7 +117,7 @@ struct skkr_ctx {
   long long dom_launches = 0;
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
   int gemm_3m = 0;     // 128-wide tiles: 1 = three-product (Gauss) complex GEMM
-  int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 128
+  int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 96 | 128
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
   int graphs = 0;
   bool capturing = false;
@@ -364,6 +364,11 @@ static void dom_collect(skkr_ctx* c) {
 // or SKKR_GEMM_TILE), else the largest tile that the shape fills.
 static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   int t = (m >= 128 && n >= 128) ? 128 : ((m > 32 || n > 32) ? 64 : 32);
+  if (t == 64) {
+    // the 96 x 96 tile where it pads much less than 64 x 64 (n = 72: 96^2 against 128^2)
+    auto area = [&](int q) { return (double)((m + q - 1) / q * q) * ((n + q - 1) / q * q); };
+    if (area(96) < 0.85 * area(64)) t = 96;
+  }
   if (c && c->tile_nn && m == c->n && n == c->n) t = c->tile_nn;
   if (c && c->tile_2nn && m == 2 * c->n && n == c->n) t = c->tile_2nn;
   if (c && c->gemm_tile_force > 0) t = c->gemm_tile_force;
@@ -390,6 +395,12 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   else if (tm == 128)  // 8-deep stages: 54 KB LDS, two work-groups per CU
     zgemm_f64_kernel<128, 64, 8><<<grid, 512, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (tm == 96 && m3)
+    zgemm_f64_kernel<96, 96, 8, 2, true><<<grid, 576, 0, c->stream>>>(
+        m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
+  else if (tm == 96)  // 9 waves, 8-deep stages (52 KB LDS)
+    zgemm_f64_kernel<96, 96, 8><<<grid, 576, 0, c->stream>>>(
         m, n, k, (const z* const*)A, lda, (const z* const*)B, ldb, C, ldc, alpha, beta, active, epi);
   else if (tm == 64 && m3)
     zgemm_f64_kernel<64, 64, 16, 2, true><<<grid, 256, 0, c->stream>>>(
@@ -742,7 +753,7 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   if (const char* e = getenv("SKKR_GEMM_3M")) c->gemm_3m = (atoi(e) != 0) ? 1 : 0;
   if (const char* e = getenv("SKKR_GEMM_TILE")) {
     const int t = atoi(e);
-    if (t == 32 || t == 64 || t == 128) c->gemm_tile_force = t;
+    if (t == 32 || t == 64 || t == 96 || t == 128) c->gemm_tile_force = t;
   }
   c->n = cfg->lk * cfg->sites_per_pl;
   c->nn = c->n * c->n;
@@ -1202,7 +1213,7 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     return SKKR_OK;
   }
   if (strcmp(key, "gemm_tile") == 0) {
-    if (value != 0 && value != 32 && value != 64 && value != 128) return SKKR_ERR_ARG;
+    if (value != 0 && value != 32 && value != 64 && value != 96 && value != 128) return SKKR_ERR_ARG;
     c->gemm_tile_force = value;
     drop_graphs(c);
     return SKKR_OK;

@@ -37,7 +37,7 @@ def small_sys(lk=4, s=2, P=3, tmodel="herm", E=0.3 + 0.1j, r2max=2):
     return S.make_system(cfg)
 
 
-@pytest.mark.parametrize("n", [9, 18, 36, 64, 128])
+@pytest.mark.parametrize("n", [9, 18, 36, 64, 65, 72, 96, 100, 128])
 def test_zgemm_matches_numpy(gpu, n):
     rng = np.random.default_rng(n)
     a = rng.normal(size=(5, n, n)) + 1j * rng.normal(size=(5, n, n))
@@ -46,6 +46,25 @@ def test_zgemm_matches_numpy(gpu, n):
         c = ctx.zgemm(a, b)
     ref = a @ b
     assert rel(c, ref) < 1e-14
+
+
+@pytest.mark.parametrize("tile", [32, 64, 96, 128])
+def test_zgemm_every_tile_on_ragged_batches(gpu, tile):
+    """Each work-group tile forced on a ragged shape (n = 72 and 200) and a
+    batch of 11, so that the XCD renumbering (8 products at a time, the rest
+    plain) meets several tiles per product."""
+    rng = np.random.default_rng(tile)
+    for n in (72, 200):
+        a = rng.normal(size=(11, n, n)) + 1j * rng.normal(size=(11, n, n))
+        b = rng.normal(size=(11, n, n)) + 1j * rng.normal(size=(11, n, n))
+        with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+            ctx.set_option("gemm_tile", tile)
+            c = ctx.zgemm(a, b)
+            ctx.set_option("gemm_3m", 1)
+            c3 = ctx.zgemm(a, b)
+        ref = a @ b
+        assert rel(c, ref) < 1e-14
+        assert rel(c3, ref) < 1e-14
 
 
 @pytest.mark.parametrize("n", [64, 128, 200])
@@ -414,6 +433,24 @@ def test_physical_tau_matches_oracle(gpu, lk, s, P, tmodel):
     tr_bz = np.array([[np.trace(phys_ref[i][np.ix_(lidx == l, lidx == l)]) for l in range(nl)]
                       for i in range(ns)])
     assert rel(np.einsum("k,kil->il", w, r["taukdiag"]), tr_bz) < TOL
+
+
+@pytest.mark.parametrize("lk,s", [(18, 4), (24, 4)])
+def test_kloop_in_the_96_tile_range_matches_oracle(gpu, lk, s):
+    """n = 72 and 96: the products run in 96 x 96 work-group tiles (9 waves,
+    whose norm reduction folds the threads above 512 first) and the
+    inversions in the 80- and 96-row kernels; tau and the decimation
+    iteration counts equal the oracle's."""
+    sy = small_sys(lk=lk, s=s, P=3, r2max=4, E=0.35 + 0.04j)
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(4)
+    ref = oracle.tau_bz(sy, E, k, w, nworkers=8)
+    print("k-loop n =", lk * s, "P = 3, 16 k", flush=True)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, k, w)
+    np.testing.assert_array_equal(r["iters"], ref["iters"])
+    assert rel(r["tau_bz"], ref["tau_bz"]) < TOL
 
 
 def test_physical_tau_adds_up_over_calls(gpu):
