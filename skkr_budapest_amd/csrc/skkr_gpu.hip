@@ -1590,6 +1590,9 @@ struct skkr_str {
     *d_g2d = nullptr;
   int* d_status = nullptr;
   PtrArr p_g, p_g1, p_gs;
+  // the rows R of the middle three PLs (planes nextra .. nextra + 2) of g, the
+  // columns R of (1 - alpha g)^-1 and the blocks (R, R) and (R, all) of gs
+  PtrArr p_gR, p_g1C, p_gsRC, p_gsR;
 };
 
 namespace {
@@ -1771,6 +1774,14 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
       (r = str_ptrs(sc, &sc->p_g1, sc->d_g1, NN, B)) || (r = str_ptrs(sc, &sc->p_gs, sc->d_gs, NN, B)) ||
       (r = ensure_inverse(sc->inv, B)))
     return fail(r);
+  {
+    const size_t r0 = (size_t)cfg->nextra * s * nl2;
+    if ((r = str_ptrs(sc, &sc->p_gR, sc->d_g + r0, NN, B)) ||
+        (r = str_ptrs(sc, &sc->p_g1C, sc->d_g1 + r0 * N, NN, B)) ||
+        (r = str_ptrs(sc, &sc->p_gsRC, sc->d_gs + r0 + r0 * N, NN, B)) ||
+        (r = str_ptrs(sc, &sc->p_gsR, sc->d_gs + r0, NN, B)))
+      return fail(r);
+  }
   if (N > 256) {
     const size_t h1 = 256, h2 = N - 256;
     if ((r = str_alloc(sc, &sc->d_X11, B * h1 * h1)) || (r = str_alloc(sc, &sc->d_C, B * h2 * h1)) ||
@@ -1853,9 +1864,12 @@ static int str_inverse_large(skkr_str* sc, int nb) {
 }
 
 // screened constants of nb <= sc->B k-points (device k) into sc->d_gs, on
-// stream; status per k in sc->d_status
+// stream; status per k in sc->d_status.  Only the rows R of the middle three
+// PLs are formed (gs[R, C] = g[R, :] (1 - alpha g)^-1[:, C]): against every
+// column for gstore's g2d, against R alone when blocks_only (the k-loop's PL
+// blocks).  The other entries of d_gs are left as they were.
 static int str_run(skkr_str* sc, hipStream_t stream, double e_re, double e_im, const z* alpha, const double* dk,
-                   int nb) {
+                   int nb, bool blocks_only) {
   const int lmax = sc->cfg.lmax, s = sc->cfg.s, nl2 = sc->nl2, N = sc->N;
   kkr::StrArgs a;
   memset(&a, 0, sizeof(a));
@@ -1923,9 +1937,13 @@ static int str_run(skkr_str* sc, hipStream_t stream, double e_re, double e_im, c
     return run_inverse(sc->inv, ia, nb);
   }()
                      : str_inverse_large(sc, nb);
-  if (r == SKKR_OK)
-    r = launch_zgemm(sc->inv, N, N, N, sc->p_g.d, N, sc->p_g1.d, N, sc->p_gs.d, N, make_double2(1, 0),
-                     make_double2(0, 0), nb, nullptr);
+  if (r == SKKR_OK) {
+    const int mr = 3 * s * nl2;
+    r = blocks_only ? launch_zgemm(sc->inv, mr, mr, N, sc->p_gR.d, N, sc->p_g1C.d, N, sc->p_gsRC.d, N,
+                                   make_double2(1, 0), make_double2(0, 0), nb, nullptr)
+                    : launch_zgemm(sc->inv, mr, N, N, sc->p_gR.d, N, sc->p_g1.d, N, sc->p_gsR.d, N,
+                                   make_double2(1, 0), make_double2(0, 0), nb, nullptr);
+  }
   sc->inv->stream = own;
   if (sc->invS) sc->invS->stream = ownS;
   return r;
@@ -1942,7 +1960,7 @@ static int str_blocks(skkr_ctx* c, int nb) {
     std::complex<double> E(c->e_re, c->e_im), psq = E;
     if (c->str_rel == 1) psq = E + E * E / (274.072 * 274.072);
     const std::complex<double> fac = psq / E;
-    CHK(str_run(sc, c->stream, psq.real(), psq.imag(), c->strAlpha, c->dk + 2 * (size_t)k0, nbb));
+    CHK(str_run(sc, c->stream, psq.real(), psq.imag(), c->strAlpha, c->dk + 2 * (size_t)k0, nbb, true));
     const size_t nn = c->nn, o = (size_t)k0 * nn;
     dim3 grid((unsigned)((nn + 255) / 256), nbb);
     if (c->str_rel)
@@ -2021,7 +2039,7 @@ int skkr_gstore_batch(skkr_str* sc, double e_re, double e_im, const double* alph
   for (int k0 = 0; k0 < nk; k0 += sc->B) {
     const int nb = std::min(sc->B, nk - k0);
     HIPCHK(hipMemcpyAsync(sc->d_k, k + 2 * (size_t)k0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, sc->stream));
-    CHK(str_run(sc, sc->stream, e_re, e_im, sc->d_alpha, sc->d_k, nb));
+    CHK(str_run(sc, sc->stream, e_re, e_im, sc->d_alpha, sc->d_k, nb, false));
     {
       dim3 grid((unsigned)((per + 255) / 256), nb);
       kkr::kkr_extract_kernel<<<grid, 256, 0, sc->stream>>>(N, nl2, s, sc->cfg.nextra, sc->d_gs, sc->d_g2d, nb);
