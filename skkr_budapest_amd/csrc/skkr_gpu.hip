@@ -430,7 +430,7 @@ static int launch_gjinv_blk(const InvArgs& a, int nb, hipStream_t st) {
   switch ((a.n + 15) / 16) {
     case 1:
     case 2: gjinv_blk_f64_kernel<2, 2, 2, PIV><<<nb, 64, 0, st>>>(a); break;
-    case 3:  // a T = 3 instance spills to scratch
+    case 3:  // <3,1,3> spills; <3,1,1> (9 waves) measured no faster than 64 rows at n = 36
     case 4: gjinv_blk_f64_kernel<4, 2, 2, PIV><<<nb, 256, 0, st>>>(a); break;
     case 5: gjinv_blk_f64_kernel<5, 1, 5, PIV><<<nb, 320, 0, st>>>(a); break;
     case 6: gjinv_blk_f64_kernel<6, 2, 3, PIV><<<nb, 384, 0, st>>>(a); break;
