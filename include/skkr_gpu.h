@@ -142,7 +142,7 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
 /* Options of a context (no reference counterpart; execution choices that
  * leave FP64 results bitwise unchanged):
  *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it
- *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 128
+ *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 96 | 128
  *   "inv_kernel" 0 blocked MFMA Gauss-Jordan | 1 row-wise Gauss-Jordan
  *                (different rounding; not bitwise equal to 0) | 2 blocked
  *                without row exchanges inside the 16x16 diagonal blocks: for
