@@ -613,7 +613,25 @@ __global__ __launch_bounds__(256) void bz_accumulate_kernel(
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= ne) return;
   double2 s = acc[e];
-  for (int b = 0; b < nb; ++b) {
+  // ne is small (sites x lk^2, a few work-groups): issue 16 loads before the
+  // adds, which stay in k order, so the sum is the same as one k at a time
+  constexpr int U = 16;
+  int b = 0;
+  for (; b + U <= nb; b += U) {
+    double2 t[U];
+    double wb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      t[u] = tau[(long long)(b + u) * stride + e];
+      wb[u] = w[b + u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      s.x += wb[u] * t[u].x;
+      s.y += wb[u] * t[u].y;
+    }
+  }
+  for (; b < nb; ++b) {
     double2 t = tau[(long long)b * stride + e];
     s.x += w[b] * t.x;
     s.y += w[b] * t.y;
