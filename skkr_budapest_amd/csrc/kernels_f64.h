@@ -650,14 +650,25 @@ __global__ __launch_bounds__(256) void pair_accumulate_kernel(
   const int a = e % lk, c = e / lk;
   const long long off = (long long)(i * lk + a) + (long long)(j * lk + c) * n;
   double2 s = acc[e];
-  for (int b = 0; b < nb; ++b) {
+  auto add = [&](int b, double2 t) {
     double sn, cs;
     sincos(kpts[2 * b] * rx + kpts[2 * b + 1] * ry, &sn, &cs);
-    const double2 t = T[b][off];
     const double pr = w[b] * cs, pi = w[b] * sgn * sn;
     s.x += pr * t.x - pi * t.y;
     s.y += pr * t.y + pi * t.x;
+  };
+  // as in bz_accumulate_kernel: 8 loads (through the pointer array) ahead of
+  // their adds, which stay in k order
+  constexpr int U = 8;
+  int b = 0;
+  for (; b + U <= nb; b += U) {
+    double2 t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = T[b + u][off];
+#pragma unroll
+    for (int u = 0; u < U; ++u) add(b + u, t[u]);
   }
+  for (; b < nb; ++b) add(b, T[b][off]);
   acc[e] = s;
 }
 
