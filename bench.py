@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("BZ-integrated tau k-points/s per energy: mixed+refine vs FP64, "
           "at 1e-8 rel err")
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0  # dense BF16 MFMA, MI355X datasheet
 # HBM bytes moved per algorithmic byte by zgemm_f64_kernel<128,128,16>: rocprofv3
 # FETCH_SIZE + WRITE_SIZE of 1024 x complex 128^3 products, calibrated with a
 # plain 256 MiB copy (profiles/hbm_traffic_pmc.md): 805.8 MB / 805.3 MB
@@ -280,6 +281,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="time only the FP64 k-loop (default: FP64, then the split-BF16 mixed mode)")
     ap.add_argument("--kkr", action="store_true",
                     help="k-loop on the layer structure-constant producer (Nb(110) stack)")
     ap.add_argument("--kkr-sample", action="store_true",
@@ -313,27 +316,25 @@ def main():
     print(f"bench: config {args.config} n={cfg.n} P={cfg.P} nk={nk} E={E}", file=sys.stderr,
           flush=True)
 
-    ctx = gpu.Context.for_system(sy)
-    ctx.set_tinv(*sy.tinv(E))
-    for i in range(args.warmup):
-        r = ctx.tau_bz(E, k, w)
-        print(f"bench: warmup {i} {r['stats']['ms_total']:.1f} ms", file=sys.stderr, flush=True)
-    times, stats = [], []
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        r = ctx.tau_bz(E, k, w, timing_detail=True)
-        times.append(time.perf_counter() - t0)
-        stats.append(r["stats"])
-        print(f"bench: step {i} {times[-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr,
-              flush=True)
-    ms = 1e3 * float(np.mean(times))
-    st = stats[-1]
-    it_l, it_r = st["mean_iter_l"], st["mean_iter_r"]
-    dom_launch_ms = sum(s["dom_ms"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
-    dom_flops = sum(s["dom_flops"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
-    achieved = dom_flops / (dom_launch_ms * 1e-3) / 1e12 if dom_launch_ms > 0 else 0.0
-    dom_bytes = sum(s["dom_bytes"] for s in stats) / max(1, sum(s["dom_launches"] for s in stats))
-    value = nk / (ms * 1e-3)
+    modes = [("fp64", gpu.FP64)] + ([] if args.fp64_only else [("mixed_bf16", gpu.MIXED_BF16)])
+    runs = {}
+    for name, prec in modes:
+        ctx = gpu.Context.for_system(sy, precision=prec)
+        ctx.set_tinv(*sy.tinv(E))
+        runs[name] = kloop_steps(ctx, E, k, w, args, name)
+        ctx.close()
+    head = runs["mixed_bf16"] if "mixed_bf16" in runs else runs["fp64"]
+    value = nk / (head["ms"] * 1e-3)
+    cfg_out = {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} "
+                           f"n={cfg.n} N_R={cfg.n_R} E={E} {cfg.mesh}x{cfg.mesh} k",
+               "precision": "mixed_bf16" if "mixed_bf16" in runs else "fp64", "nk": nk}
+    for name, r in runs.items():
+        cfg_out[name] = mode_summary(r, nk, cfg)
+    if "mixed_bf16" in runs:
+        from skkr_budapest_amd import observables as O
+        cfg_out["mixed_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_bf16"]["ms"]
+        cfg_out["mixed_error_vs_fp64"] = O.error_report(runs["mixed_bf16"]["tau_bz"], runs["fp64"]["tau_bz"])
+    rl = rooflines(head)
     out = {
         "metric": METRIC,
         "value": value,
@@ -341,33 +342,81 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic (seeded G_r(R), t^-1)",
-        "config": {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} "
-                               f"n={cfg.n} N_R={cfg.n_R} E={E} {cfg.mesh}x{cfg.mesh} k",
-                   "precision": "fp64", "nk": nk,
-                   "mean_iter_l": it_l, "mean_iter_r": it_r,
-                   "algorithmic_tflops": nk * algorithmic_flops_per_k(cfg.n, cfg.P, it_l, it_r)
-                   / (ms * 1e-3) / 1e12,
-                   "phase_ms": {p: st[f"ms_{p}"] for p in
-                                ("fourier", "decimation", "sweeps", "accumulate")}},
-        "roofline": {"bound": "mfma", "kernel": st["dom_name"], "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": dom_bytes * GEMM_TRAFFIC_PER_ALGO_BYTE,
-                     "traffic_unit": "bytes per launch",
-                     "algorithmic_flops_per_launch": dom_flops,
-                     "avg_launch_us": dom_launch_ms * 1e3},
+        "config": cfg_out,
+        "roofline": rl[0],
+        "rooflines": rl,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sy, E, k, w)
-    ctx.close()
     print(json.dumps(out))
+
+
+def kloop_steps(ctx, E, k, w, args, name):
+    for i in range(args.warmup):
+        r = ctx.tau_bz(E, k, w)
+        print(f"bench: {name} warmup {i} {r['stats']['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    times, stats = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        r = ctx.tau_bz(E, k, w, timing_detail=True)
+        times.append(time.perf_counter() - t0)
+        stats.append(r["stats"])
+        print(f"bench: {name} step {i} {times[-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr,
+              flush=True)
+    return {"ms": 1e3 * float(np.mean(times)), "stats": stats, "tau_bz": r["tau_bz"]}
+
+
+def mode_summary(r, nk, cfg):
+    st = r["stats"][-1]
+    it_l, it_r = st["mean_iter_l"], st["mean_iter_r"]
+    return {"k_points_per_s": nk / (r["ms"] * 1e-3), "ms_per_step": r["ms"],
+            "mean_iter_l": it_l, "mean_iter_r": it_r, "n_launches": st["n_launches"],
+            "algorithmic_tflops": nk * algorithmic_flops_per_k(cfg.n, cfg.P, it_l, it_r)
+            / (r["ms"] * 1e-3) / 1e12,
+            "phase_ms": {p: st[f"ms_{p}"] for p in ("fourier", "decimation", "sweeps", "accumulate")}}
+
+
+def rooflines(run):
+    """Roofline entries of the GEMM and inversion kernels of the run, the one
+    with more device time first.  achieved = algorithmic flops (active
+    decimation chains only) / summed HIP-event durations of its launches."""
+    stats = run["stats"]
+    out = []
+    for cls in ("dom", "inv"):
+        launches = sum(s[f"{cls}_launches"] for s in stats)
+        ms = sum(s[f"{cls}_ms"] for s in stats)
+        flops = sum(s[f"{cls}_flops"] for s in stats)
+        nbytes = sum(s[f"{cls}_bytes"] for s in stats)
+        if not launches or ms <= 0:
+            continue
+        alg = flops / (ms * 1e-3) / 1e12
+        per = flops / launches
+        if cls == "dom" and stats[-1]["dom_name"].startswith("zgemm_split"):
+            exe = sum(s["dom_flops_exec"] for s in stats) / (ms * 1e-3) / 1e12
+            ent = {"bound": "mfma", "kernel": "zgemm_split_kernel", "achieved": exe,
+                   "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": exe / PEAK_BF16_TFLOPS,
+                   "achieved_is": "executed BF16 MFMA flops",
+                   "algorithmic_tflops_fp64_equivalent": alg,
+                   "algorithmic_frac_of_fp64_peak": alg / PEAK_FP64_TFLOPS}
+        else:
+            name = stats[-1]["dom_name"] if cls == "dom" else "gjinv_blk_f64_kernel"
+            ent = {"bound": "mfma", "kernel": name, "achieved": alg, "peak": PEAK_FP64_TFLOPS,
+                   "unit": "TFLOP/s", "frac": alg / PEAK_FP64_TFLOPS}
+        ent.update({"traffic": (nbytes / launches) * (GEMM_TRAFFIC_PER_ALGO_BYTE if cls == "dom" else 1.0),
+                    "traffic_unit": "algorithmic bytes per launch x PMC calibration" if cls == "dom"
+                    else "algorithmic bytes per launch",
+                    "algorithmic_flops_per_launch": per, "avg_launch_us": 1e3 * ms / launches,
+                    "launches": launches, "device_ms": ms})
+        out.append(ent)
+    out.sort(key=lambda e: -e["device_ms"])
+    return out
 
 
 if __name__ == "__main__":

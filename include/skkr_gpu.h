@@ -42,11 +42,20 @@ enum {
                                 (kamstr.f:358-359,388-389, directg.f:128-131) */
 };
 
-/* precision modes */
+/* precision modes (no reference counterpart; the reference is FP64 only) */
 enum {
   SKKR_FP64 = 0,        /* every product and inversion in FP64 */
-  SKKR_MIXED_FP32 = 1,  /* FP32 MFMA products/inversions + FP64 refinement */
-  SKKR_MIXED_BF16 = 2   /* split-BF16 MFMA products + FP64 refinement */
+  SKKR_MIXED_FP32 = 1,  /* not built: FP32 products lose ~5 digits in the
+                           decimation (SKKR_ERR_UNSUPPORTED) */
+  SKKR_MIXED_BF16 = 2   /* split-BF16 products: every complex GEMM of the
+                           decimation and the PL sweeps of skkr_tau_bz* as
+                           exact BF16 slice products (Ozaki scheme) summed in
+                           FP32 and recombined in FP64 -- S = 5 slices per
+                           operand (2^-44 relative per product) in the
+                           decimation and the sweeps; inversions and the BZ
+                           sum in FP64; for Im E < fp64_dec_below_imE the
+                           decimation runs in FP64.  The per-routine entry
+                           points stay FP64. */
 };
 
 typedef struct skkr_ctx skkr_ctx;
@@ -83,6 +92,13 @@ typedef struct {
   double dom_flops;       /* algorithmic flops of those launches */
   char dom_name[32];
   double dom_bytes;       /* algorithmic HBM bytes of those launches (A, B read, C written) */
+  double dom_flops_exec;  /* SKKR_MIXED_BF16: BF16 MFMA flops the split products
+                             executed (slice pairs x 4 real products, padded tiles) */
+  /* inversion launches (gjinv kernels), timed like the GEMMs; in skkr_tau_bz*
+   * the flops and bytes of both classes count only the decimation chains that
+   * were still iterating (n <= 128) */
+  long long inv_launches;
+  double inv_ms, inv_flops, inv_bytes;
 } skkr_stats;
 
 /* Context: one per host thread / rank; calls on a context are serialised on
@@ -151,7 +167,13 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *                or nopivot in the environment)
  *   "gemm_3m"   0 four-product | 1 three-product (Gauss) complex GEMM on the
  *               128 tile: 3/4 of the MFMA work, normwise FP64 accuracy,
- *               different rounding (also SKKR_GEMM_3M=1 in the environment) */
+ *               different rounding (also SKKR_GEMM_3M=1 in the environment)
+ *   "split_dec", "split_sw", "split_sw_low"  SKKR_MIXED_BF16: slices per
+ *               operand (2..5, or 0 for FP64 products) of the decimation, of
+ *               the sweeps, and of the sweeps below fp64_dec_below_imE
+ *               (defaults 5, 5, 5)
+ *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
+ *               that many slices (0: FP64) */
 int skkr_set_option(skkr_ctx *ctx, const char *key, int value);
 
 typedef struct {

@@ -23,6 +23,7 @@
 #include "../../include/skkr_gpu.h"
 #include "kernels_f64.h"
 #include "gjinv_blk.h"
+#include "zgemm_split.h"
 #include "kkrlat.h"
 #include "kkrstr.h"
 #include "cpa.h"
@@ -112,15 +113,27 @@ struct skkr_ctx {
   std::vector<hipEvent_t> evpool;  // pairs bracketing each dominant-kernel launch
   hipEvent_t phase_ev[5] = {};     // phase boundaries of a tau_bz call, tile timing
   std::vector<double> evflops, evbytes;
+  std::vector<int> evclass;  // 0 GEMM, 1 inversion
   size_t evused = 0;
-  double dom_ms = 0.0, dom_flops = 0.0, dom_bytes = 0.0;
+  double dom_ms = 0.0, dom_flops = 0.0, dom_bytes = 0.0;  // GEMM launches
   long long dom_launches = 0;
+  double inv_ms = 0.0, inv_flops = 0.0, inv_bytes = 0.0;  // inversion launches
+  long long inv_launches = 0;
+  int dec_rounds = 0;  // iteration rounds of the last run_decimation
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
   int gemm_3m = 0;     // 128-wide tiles: 1 = three-product (Gauss) complex GEMM
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 96 | 128
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
   int graphs = 0;
   bool capturing = false;
+  // split-BF16 products (SKKR_MIXED_BF16): slices of the GEMMs being launched
+  // (0 = FP64 MFMA kernel), and the choices per phase
+  int split = 0;
+  int split_dec = 5;      // decimation and lead self-energies, Im E >= fp64_dec_below_imE
+  int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
+  int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
+  int zgemm_split = 0;    // skkr_zgemm_batch: 0 FP64, else slices
+  double dom_flops_exec = 0.0;  // executed BF16 MFMA flops of the split launches
   // screening of the interface sites (physical tau, phystau.f)
   bool have_screening = false;
   int nl = 0;
@@ -130,7 +143,7 @@ struct skkr_ctx {
   z* taubz_phys = nullptr;
   z* phk = nullptr;      // per-k per-site per-l traces of a batch
   int phk_cap = 0;
-  std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by batch size nb
+  std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by 8 nb + split
   // structure constants from the layer producer instead of the Fourier sum
   skkr_str* str = nullptr;
   z* strAlpha = nullptr;          // (lmax+1) x s screening parameters
@@ -338,11 +351,12 @@ static void dom_begin(skkr_ctx* c) {
   }
   hipEventRecord(c->evpool[c->evused], c->stream);
 }
-static void dom_end(skkr_ctx* c, double flops, double bytes) {
+static void dom_end(skkr_ctx* c, double flops, double bytes, int cls = 0) {
   if (!c->timing_detail || c->capturing) return;
   hipEventRecord(c->evpool[c->evused + 1], c->stream);
   c->evflops.push_back(flops);
   c->evbytes.push_back(bytes);
+  c->evclass.push_back(cls);
   c->evused += 2;
 }
 // call after the stream was synchronised
@@ -350,14 +364,22 @@ static void dom_collect(skkr_ctx* c) {
   for (size_t i = 0; i + 1 < c->evused; i += 2) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]);
-    c->dom_ms += ms;
-    c->dom_flops += c->evflops[i / 2];
-    c->dom_bytes += c->evbytes[i / 2];
-    c->dom_launches++;
+    if (c->evclass[i / 2] == 0) {
+      c->dom_ms += ms;
+      c->dom_flops += c->evflops[i / 2];
+      c->dom_bytes += c->evbytes[i / 2];
+      c->dom_launches++;
+    } else {
+      c->inv_ms += ms;
+      c->inv_flops += c->evflops[i / 2];
+      c->inv_bytes += c->evbytes[i / 2];
+      c->inv_launches++;
+    }
   }
   c->evused = 0;
   c->evflops.clear();
   c->evbytes.clear();
+  c->evclass.clear();
 }
 
 // Work-group tile of a GEMM shape: the context's tuned choice (skkr_autotune
@@ -378,6 +400,35 @@ static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   *tiles = ((m + t - 1) / t) * ((n + tn - 1) / tn);
 }
 
+// Split-BF16 product (zgemm_split.h) with S = c->split slices per operand.
+static int split_tiles(int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); }
+
+static int launch_zgemm_split(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
+                              z** C, int ldc, z alpha, z beta, int nb, const int* active,
+                              const EpiArgs& epi) {
+  dim3 grid(split_tiles(m, n), nb);
+  dom_begin(c);
+  const z* const* a = (const z* const*)A;
+  const z* const* b = (const z* const*)B;
+  switch (c->split) {
+    case 2: zgemm_split_kernel<2><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 3: zgemm_split_kernel<3><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 4: zgemm_split_kernel<4><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 5: zgemm_split_kernel<5><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    default: return SKKR_ERR_UNSUPPORTED;
+  }
+  HIPCHK(hipGetLastError());
+  // executed BF16 MFMA work: S (S + 1) / 2 slice pairs x 4 real products on
+  // the padded 64 x 64 x 32 tiles
+  const double pm = (m + 63) / 64 * 64.0, pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
+  const double pairs = c->split * (c->split + 1) / 2.0;
+  if (c->timing_detail && !c->capturing) c->dom_flops_exec += pairs * 4.0 * 2.0 * pm * pn * pk * nb;
+  dom_end(c, 8.0 * m * n * (double)k * nb,
+          16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
+  c->launches++;
+  return SKKR_OK;
+}
+
 static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
                         z** C, int ldc, z alpha, z beta, int nb, const int* active,
                         const EpiArgs* epi_in = nullptr) {
@@ -385,6 +436,8 @@ static int launch_zgemm(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B,
   EpiArgs epi;
   memset(&epi, 0, sizeof(epi));
   if (epi_in) epi = *epi_in;
+  if (c && c->split > 0) return launch_zgemm_split(c, m, n, k, A, lda, B, ldb, C, ldc, alpha, beta, nb,
+                                                   active, epi);
   int tm, tiles;
   gemm_tile(c, m, n, &tm, &tiles);
   dim3 grid(tiles, nb);
@@ -442,14 +495,25 @@ static int launch_gjinv_blk(const InvArgs& a, int nb, hipStream_t st) {
   return SKKR_OK;
 }
 
+// algorithmic work of an inversion launch: 8 n^3 flops per matrix; the
+// matrix (and each subtracted term) read, the inverse written
+static void inv_end(skkr_ctx* c, const InvArgs& a, int nb) {
+  const double n = a.n;
+  const double terms = 2.0 + (a.sub1 ? 1.0 : 0.0) + (a.sub2 ? 1.0 : 0.0);
+  dom_end(c, 8.0 * n * n * n * nb, 16.0 * terms * n * n * nb, 1);
+}
+
 static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
   if (nb <= 0) return SKKR_OK;
   if (a.n < 1) return SKKR_ERR_ARG;
   if (c->inv_kernel == 0 || c->inv_kernel == 2) {  // blocked MFMA Gauss-Jordan (2: no pivoting in the 16 x 16 blocks)
+    dom_begin(c);
     CHK(c->inv_kernel == 0 ? launch_gjinv_blk<true>(a, nb, c->stream) : launch_gjinv_blk<false>(a, nb, c->stream));
+    inv_end(c, a, nb);
     c->launches++;
     return SKKR_OK;
   }
+  dom_begin(c);
   if (a.n <= 32) {
     gjinv_f64_kernel<8, 8, 4, 4><<<nb, 64, 0, c->stream>>>(a);
   } else if (a.n <= 64) {
@@ -460,6 +524,7 @@ static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
     return SKKR_ERR_UNSUPPORTED;
   }
   HIPCHK(hipGetLastError());
+  inv_end(c, a, nb);
   c->launches++;
   return SKKR_OK;
 }
@@ -470,8 +535,18 @@ static int launch_gjinv(skkr_ctx* c, const InvArgs& a, int nb) {
 //   C = A21 A11^-1, D = A11^-1 A12, S = A22 - C A12.
 // No rows move between the halves; the matrices inverted here have a
 // positive-definite anti-Hermitian part, so A11 and S stay invertible.
+// Temporarily sets the slices of the context's GEMM launches (restored on
+// every return path).
+struct SplitScope {
+  skkr_ctx* c;
+  int saved;
+  SplitScope(skkr_ctx* c_, int s) : c(c_), saved(c_->split) { c->split = s; }
+  ~SplitScope() { c->split = saved; }
+};
+
 static int run_inverse(skkr_ctx* c, const InvArgs& a, int nb) {
   if (nb <= 0) return SKKR_OK;
+  SplitScope fp64(c, 0);  // inversions (and their block GEMMs) stay FP64
   if (a.n <= 128) return launch_gjinv(c, a, nb);
   if (a.n > 256 || a.mode != 0 || !c->Asc || nb > 2 * c->invB || a.n != c->n) return SKKR_ERR_UNSUPPORTED;
   const int n = a.n, h1 = 128, h2 = n - 128;
@@ -565,8 +640,11 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   int tm, tiles;
   gemm_tile(c, 2 * n, n, &tm, &tiles);
+  if (c->split > 0) tiles = split_tiles(2 * n, n);  // work-groups of the split kernel
   int cur = 0;
+  c->dec_rounds = 0;
   for (int iter = 1; iter <= itermax + 1; ++iter) {
+    c->dec_rounds = iter;
     InvArgs ia = inv_args(c);
     ia.base = c->p_eps.d;
     ia.out = c->p_Yinv.d;
@@ -684,7 +762,8 @@ static int sweep_launches(skkr_ctx* c, int nb) {
 // buffers and pointer arrays are fixed for the context's batch size).
 static int run_sweeps(skkr_ctx* c, int nb) {
   if (!c->graphs || c->pairs_on || c->layered) return sweep_launches(c, nb);
-  auto it = c->sweep_graphs.find(nb);
+  const int key = 8 * nb + c->split;
+  auto it = c->sweep_graphs.find(key);
   if (it == c->sweep_graphs.end()) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -701,7 +780,7 @@ static int run_sweeps(skkr_ctx* c, int nb) {
     const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
     HIPCHK(ei);
-    it = c->sweep_graphs.emplace(nb, ge).first;
+    it = c->sweep_graphs.emplace(key, ge).first;
   }
   HIPCHK(hipGraphLaunch(it->second, c->stream));
   return SKKR_OK;
@@ -741,7 +820,7 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   if (cfg->lk <= 0 || cfg->sites_per_pl <= 0 || cfg->n_pl <= 0 || cfg->errmax <= 0 ||
       cfg->itermax_l < 1 || cfg->itermax_r < 1)
     return SKKR_ERR_ARG;
-  if (cfg->precision != SKKR_FP64) return SKKR_ERR_UNSUPPORTED;
+  if (cfg->precision != SKKR_FP64 && cfg->precision != SKKR_MIXED_BF16) return SKKR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
     return SKKR_ERR_HIP;
@@ -1005,10 +1084,24 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   c->dom_flops = 0;
   c->dom_launches = 0;
   c->dom_bytes = 0;
+  c->inv_ms = 0;
+  c->inv_flops = 0;
+  c->inv_launches = 0;
+  c->inv_bytes = 0;
   c->evused = 0;
   c->evflops.clear();
   c->evbytes.clear();
+  c->evclass.clear();
   c->launches = 0;
+  c->dom_flops_exec = 0;
+  // mixed precision (SKKR_MIXED_BF16): split-BF16 products in the decimation
+  // and the sweeps; the decimation stays FP64 for Im E below the threshold
+  int s_dec = 0, s_sw = 0;
+  if (c->cfg.precision == SKKR_MIXED_BF16) {
+    const bool near_axis = c->e_im < c->cfg.fp64_dec_below_imE;
+    s_dec = near_axis ? 0 : c->split_dec;
+    s_sw = near_axis ? c->split_sw_low : c->split_sw;
+  }
   HIPCHK(hipMemsetAsync(c->taubz, 0, sizeof(z) * tsz, c->stream));
   CHK(ensure_phase_events(c));
   hipEvent_t* ev = c->phase_ev;
@@ -1043,6 +1136,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     }
     hipEventRecord(ev[1], c->stream);
     if (!reuse) {
+      SplitScope sd(c, s_dec);
       CHK(run_decimation(c, true, true, c->str ? &c->p_mt : nullptr));
       if (c->cpa_mode == 1)
         HIPCHK(hipMemcpyAsync(c->Y1s, c->Ycur, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
@@ -1050,7 +1144,10 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
       HIPCHK(hipMemcpyAsync(c->Ycur, c->Y1s, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
     }
     hipEventRecord(ev[2], c->stream);
-    CHK(run_sweeps(c, nb));
+    {
+      SplitScope ss(c, s_sw);
+      CHK(run_sweeps(c, nb));
+    }
     if (c->pairs_on) CHK(pair_sums(c, nb));
     hipEventRecord(ev[3], c->stream);
     {
@@ -1092,6 +1189,28 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     hipEventElapsedTime(&t, ev[1], ev[2]); ms_d += t;
     hipEventElapsedTime(&t, ev[2], ev[3]); ms_s += t;
     hipEventElapsedTime(&t, ev[3], ev[4]); ms_a += t;
+    if (c->timing_detail && !reuse && c->n <= 128) {
+      // The decimation launches cover all 2B chains; only the active ones do
+      // work.  Replace the nominal counts of those launches by the work of
+      // the chains that ran: chain j took part in iterations 1..iters_j
+      // (three 2n x n x n products and one inversion each), every started
+      // chain in the final inversion and the two n^3 self-energy products.
+      const double n3 = (double)c->n * c->n * c->n, n2 = (double)c->n * c->n;
+      const double B2d = 2.0 * B, rounds = c->dec_rounds;
+      double it_sum = 0.0;
+      for (int b = 0; b < nb; ++b) it_sum += h_it[b] + h_it[B + b];
+      const double started = 2.0 * nb;
+      const double corr = (it_sum - rounds * B2d) * 3.0 * 16.0 * n3 + (started - B2d) * 2.0 * 8.0 * n3;
+      c->dom_flops += corr;
+      if (s_dec > 0) {  // executed split work of the same launches (64 x 64 x 32 padding)
+        const double pad = (double)((2 * c->n + 63) / 64 * 64) * ((c->n + 63) / 64 * 64) * ((c->n + 31) / 32 * 32) /
+                           (2.0 * n3);
+        c->dom_flops_exec += corr * s_dec * (s_dec + 1) / 2.0 * pad;
+      }
+      c->dom_bytes += (it_sum - rounds * B2d) * 3.0 * 80.0 * n2 + (started - B2d) * 2.0 * 48.0 * n2;
+      c->inv_flops += (it_sum - rounds * B2d) * 8.0 * n3 + (started - B2d) * 8.0 * n3;
+      c->inv_bytes += (it_sum - rounds * B2d) * 32.0 * n2 + (started - B2d) * 32.0 * n2;
+    }
     for (int b = 0; b < nb; ++b) {
       int il = h_it[b], ir = h_it[B + b];
       sum_l += il;
@@ -1157,7 +1276,13 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     st->dom_ms = c->dom_ms;
     st->dom_flops = c->dom_flops;
     st->dom_bytes = c->dom_bytes;
-    snprintf(st->dom_name, sizeof(st->dom_name), "zgemm_f64_kernel");
+    st->dom_flops_exec = c->dom_flops_exec;
+    st->inv_launches = c->inv_launches;
+    st->inv_ms = c->inv_ms;
+    st->inv_flops = c->inv_flops;
+    st->inv_bytes = c->inv_bytes;
+    snprintf(st->dom_name, sizeof(st->dom_name), "%s",
+             (s_dec > 0 && s_sw > 0) ? "zgemm_split_kernel" : (s_sw > 0 ? "zgemm_f64+split" : "zgemm_f64_kernel"));
   }
   c->timing_detail = 0;
   return rc;
@@ -1222,6 +1347,15 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     if (value < 0 || value > 2) return SKKR_ERR_ARG;
     c->inv_kernel = value;
     drop_graphs(c);
+    return SKKR_OK;
+  }
+  if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0 ||
+      strcmp(key, "zgemm_split") == 0) {
+    if (value != 0 && (value < 2 || value > 5)) return SKKR_ERR_ARG;
+    if (strcmp(key, "split_dec") == 0) c->split_dec = value;
+    else if (strcmp(key, "split_sw") == 0) c->split_sw = value;
+    else if (strcmp(key, "split_sw_low") == 0) c->split_sw_low = value;
+    else c->zgemm_split = value;
     return SKKR_OK;
   }
   if (strcmp(key, "gemm_3m") == 0) {
@@ -1374,6 +1508,7 @@ int skkr_zgemm_batch(skkr_ctx* c, int n, int nb, const double* a, const double* 
   CHK(A.init(nb, nn, a, c->stream));
   CHK(Bm.init(nb, nn, b, c->stream));
   CHK(C.init(nb, nn, nullptr, c->stream));
+  SplitScope sp(c, c->zgemm_split);
   CHK(launch_zgemm(c, n, n, n, A.p, n, Bm.p, n, C.p, n, make_double2(1, 0), make_double2(0, 0), nb, nullptr));
   HIPCHK(hipMemcpyAsync(cc, C.d, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -27,7 +27,9 @@ module skkr_gpu
      integer(c_long_long) :: dom_launches
      real(c_double) :: dom_ms, dom_flops
      character(kind=c_char) :: dom_name(32)
-     real(c_double) :: dom_bytes
+     real(c_double) :: dom_bytes, dom_flops_exec
+     integer(c_long_long) :: inv_launches
+     real(c_double) :: inv_ms, inv_flops, inv_bytes
   end type skkr_stats
 
   ! layer structure-constant producer (gstore.f with bulkgeo)

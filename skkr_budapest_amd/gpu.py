@@ -61,7 +61,9 @@ class Stats(ctypes.Structure):
                 ("n_launches", ctypes.c_longlong), ("timing_detail", ctypes.c_int),
                 ("dom_launches", ctypes.c_longlong), ("dom_ms", ctypes.c_double),
                 ("dom_flops", ctypes.c_double), ("dom_name", ctypes.c_char * 32),
-                ("dom_bytes", ctypes.c_double)]
+                ("dom_bytes", ctypes.c_double), ("dom_flops_exec", ctypes.c_double),
+                ("inv_launches", ctypes.c_longlong), ("inv_ms", ctypes.c_double),
+                ("inv_flops", ctypes.c_double), ("inv_bytes", ctypes.c_double)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -1,0 +1,140 @@
+"""SKKR_MIXED_BF16 on the GPU: the split-BF16 GEMM against its CPU
+emulation (bitwise), and the mixed k-loop against the FP64 oracle and the
+FP64 GPU path.
+
+Tolerance of the mixed results: relative error <= 1e-8 of the BZ-integrated
+tau (Frobenius over all sites), of tr G and of the DOS, against the FP64
+result of the same inputs -- the accuracy the mixed mode is built for."""
+import numpy as np
+import pytest
+
+import oracle
+import split_emul as se
+from skkr_budapest_amd import contour
+from skkr_budapest_amd import observables as O
+from skkr_budapest_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-8
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+@pytest.mark.parametrize("n", [36, 128, 200])
+def test_split_gemm_matches_emulation_bitwise(gpu, n):
+    """Every slice count on ragged shapes (n = 36, 200: partial 64-tiles and
+    K-chunks) and a batch of 11 (XCD renumbering of 8 products plus a plain
+    tail); rows and columns scaled over 12 orders of magnitude."""
+    rng = np.random.default_rng(1000 + n)
+    nb = 11
+    a = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    b = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    a *= np.logspace(-6, 6, n)[None, :, None]
+    b *= np.logspace(4, -8, n)[None, None, :]
+    print(f"split GEMM n={n} nb={nb}", flush=True)
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        for s in (2, 3, 4, 5):
+            ctx.set_option("zgemm_split", s)
+            c = ctx.zgemm(a, b)
+            for j in range(nb):
+                np.testing.assert_array_equal(c[j], se.split_product(a[j], b[j], s))
+        ctx.set_option("zgemm_split", 0)
+        c64 = ctx.zgemm(a, b)
+    assert np.linalg.norm(c64 - a @ b) < 1e-14 * np.linalg.norm(a @ b)
+
+
+def _errors(tau, ref):
+    return O.error_report(tau, ref)
+
+
+def test_mixed_config1_sample_matches_oracle(gpu):
+    """Config 1 (n = 128, 20 PLs, E = 0.6 + 0.02i) on 32 k-points of its
+    64 x 64 mesh: the mixed k-loop against the FP64 oracle."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.linspace(0, len(w) - 1, 32).astype(int)
+    ks, ws = k[idx], np.full(len(idx), 1.0 / len(idx))
+    print("mixed cfg1 sample: 32 k", flush=True)
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=16)
+    with gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, ks, ws)
+    assert (r["status"] == 0).all()
+    e = _errors(r["tau_bz"], ref["tau_bz"])
+    print("mixed vs oracle", e, flush=True)
+    for v in e.values():
+        assert v < MIXED_TOL
+    # per-k traces too
+    assert np.abs(r["trk"] - ref["trk"]).max() < MIXED_TOL * np.abs(ref["trk"]).max()
+    # the split products change the decimation's |alpha| only far below its
+    # stop threshold: the iteration counts are those of FP64
+    assert np.abs(r["iters"] - ref["iters"]).max() <= 1
+
+
+def test_mixed_contour_config2(gpu):
+    """Config 2: the config-1 system on the 16-point contour (Im E from 0.695
+    down to 0.005), full 64 x 64 mesh per energy, mixed against FP64 on the
+    GPU, with the oracle on 2 k-points per energy.  Below Im E = 0.02 the
+    decimation runs in FP64 (fp64_dec_below_imE)."""
+    cfg = S.CONFIGS[2]
+    sy = S.make_system(cfg)
+    z, wz = contour.config_contour()
+    k, w = S.uniform_kmesh(cfg.mesh)
+    fp, mx = [], []
+    with gpu.Context.for_system(sy) as c64, gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as cmx:
+        for ie, E in enumerate(z):
+            print(f"contour energy {ie} E={E:.4f}", flush=True)
+            for c, out in ((c64, fp), (cmx, mx)):
+                c.set_tinv(*sy.tinv(E))
+                r = c.tau_bz(E, k, w)
+                assert (r["status"] == 0).all()
+                out.append(r)
+            e = _errors(mx[-1]["tau_bz"], fp[-1]["tau_bz"])
+            print("  mixed vs FP64", e, mx[-1]["stats"]["dom_name"], flush=True)
+            for v in e.values():
+                assert v < MIXED_TOL
+            # near the axis the decimation is FP64
+            if E.imag < 0.02:
+                np.testing.assert_array_equal(mx[-1]["iters"], fp[-1]["iters"])
+            # oracle spot check of the per-k traces
+            sel = np.array([7, 2900])
+            ref = oracle.tau_bz(sy, E, k[sel], w[sel], nworkers=2)
+            assert np.abs(mx[-1]["trk"][sel] - ref["trk"]).max() < MIXED_TOL * np.abs(ref["trk"]).max()
+    g64 = O.integrated_g([O.tr_g(r["tau_bz"]) for r in fp], wz)
+    gmx = O.integrated_g([O.tr_g(r["tau_bz"]) for r in mx], wz)
+    assert abs(gmx - g64) < MIXED_TOL * abs(g64)
+
+
+def test_mixed_precision_choices(gpu):
+    """The slice counts per phase are options; the FP32 mode is not built."""
+    sy = S.make_system(S.small_config(lk=8, s=2, P=4, r2max=4, energies=(0.5 + 0.05j,)))
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(4)
+    with gpu.Context.for_system(sy) as c64:
+        c64.set_tinv(*sy.tinv(E))
+        ref = c64.tau_bz(E, k, w)
+    with gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as c:
+        c.set_tinv(*sy.tinv(E))
+        r5 = c.tau_bz(E, k, w)
+        c.set_option("split_dec", 3)
+        c.set_option("split_sw", 3)
+        r3 = c.tau_bz(E, k, w)
+        c.set_option("split_dec", 0)
+        c.set_option("split_sw", 0)
+        r0 = c.tau_bz(E, k, w)
+        with pytest.raises(gpu.SkkrError):
+            c.set_option("split_sw", 6)
+    np.testing.assert_array_equal(r0["tau_bz"], ref["tau_bz"])  # 0 slices: the FP64 kernels
+    e_def = O.rel_err(r5["tau_bz"], ref["tau_bz"])  # 5 slices in the decimation and the sweeps
+    e3 = O.rel_err(r3["tau_bz"], ref["tau_bz"])
+    print("default", e_def, "three slices", e3, flush=True)
+    assert e_def < MIXED_TOL and e3 < 1e-4 and e_def < e3 / 100
+    with pytest.raises(gpu.SkkrError):
+        gpu.Context(lk=8, sites_per_pl=2, n_pl=4, precision=gpu.MIXED_FP32)
