@@ -83,7 +83,7 @@ typedef struct {
   double mean_iter_l, mean_iter_r;
   int max_iter;
   int n_batches;
-  long long n_launches;
+  long long n_launches;   /* kernel launches and graph replays issued by the host */
   /* dominant kernel: launches and summed event time (HIP events bracketing
    * each launch on the ctx stream; filled only when timing_detail != 0) */
   int timing_detail;
@@ -157,7 +157,8 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
 
 /* Options of a context (no reference counterpart; execution choices that
  * leave FP64 results bitwise unchanged):
- *   "graphs"    0/1  capture the PL sweeps of a batch in a HIP graph, replay it
+ *   "graphs"    1/0  capture the PL sweeps of a batch in a HIP graph and replay
+ *               it (default 1; bitwise equal to the plain launches)
  *   "gemm_tile" 0 (by shape, or the autotuned choice) | 32 | 64 | 96 | 128
  *   "inv_kernel" 0 blocked MFMA Gauss-Jordan | 1 row-wise Gauss-Jordan
  *                (different rounding; not bitwise equal to 0) | 2 blocked

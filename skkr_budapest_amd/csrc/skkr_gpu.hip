@@ -88,7 +88,8 @@ struct skkr_ctx {
   int invB = 0;  // batch capacity of the 2x2 block inversion scratch
   z** dv_out[4] = {nullptr, nullptr, nullptr, nullptr};  // out + offsets (B11, B21, B12, B22)
   int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
-      *nactive = nullptr, *status_sw = nullptr;
+      *nactive = nullptr, *status_sw = nullptr, *active0 = nullptr;
+  int dec_hint = 8;  // iterations launched before the first convergence read-back
   z *Gs = nullptr, *Zs = nullptr, *Ycur = nullptr, *W = nullptr, *Zb = nullptr,
     *Xc = nullptr, *tauk = nullptr, *trk = nullptr;
   z* taubz = nullptr;
@@ -124,7 +125,7 @@ struct skkr_ctx {
   int gemm_3m = 0;     // 128-wide tiles: 1 = three-product (Gauss) complex GEMM
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 96 | 128
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
-  int graphs = 0;
+  int graphs = 1;  // sweeps of a batch replayed from a captured graph (bitwise equal to launches)
   bool capturing = false;
   // split-BF16 products (SKKR_MIXED_BF16): slices of the GEMMs being launched
   // (0 = FP64 MFMA kernel), and the choices per phase
@@ -231,7 +232,9 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_t(c, &c->iters, B2));
   CHK(dalloc_t(c, &c->status, B2));
   CHK(dalloc_t(c, &c->itermax, B2));
-  CHK(dalloc_t(c, &c->nactive, 1));
+  // still-active chain count after each decimation iteration (index = iteration)
+  CHK(dalloc_t(c, &c->nactive, (size_t)std::max(c->cfg.itermax_l, c->cfg.itermax_r) + 2));
+  CHK(dalloc_t(c, &c->active0, B2));
   CHK(dalloc_t(c, &c->status_sw, (size_t)B));
   CHK(dalloc_t(c, &c->Gs, P * B * nn));
   CHK(dalloc_t(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
@@ -633,54 +636,78 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
                                              add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
                                              c->p_eps.d, c->p_x.d);
   HIPCHK(hipGetLastError());
-  std::vector<int> act(B2);
-  HIPCHK(hipMemcpyAsync(act.data(), c->active, B2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  // the chains started (restored for the final inversion and products)
+  HIPCHK(hipMemcpyAsync(c->active0, c->active, B2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
   const int itermax = std::max(c->cfg.itermax_l, c->cfg.itermax_r);
+  HIPCHK(hipMemsetAsync(c->nactive, 0, sizeof(int) * (itermax + 2), c->stream));
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   int tm, tiles;
   gemm_tile(c, 2 * n, n, &tm, &tiles);
   if (c->split > 0) tiles = split_tiles(2 * n, n);  // work-groups of the split kernel
   int cur = 0;
   c->dec_rounds = 0;
-  for (int iter = 1; iter <= itermax + 1; ++iter) {
-    c->dec_rounds = iter;
-    InvArgs ia = inv_args(c);
-    ia.base = c->p_eps.d;
-    ia.out = c->p_Yinv.d;
-    ia.active = c->active;
-    ia.status = c->status;
-    CHK(run_inverse(c, ia, B2));
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
-                     one, zero, B2, c->active));
-    EpiArgs e;
-    memset(&e, 0, sizeof(e));
-    e.n = n;
-    e.eps = c->p_eps.d;
-    e.x = c->p_x.d;
-    e.abn = c->p_AB[cur ^ 1].d;
-    e.norm = c->dnorm;
-    e.tiles = tiles;
-    e.mode = EPI_DEC_BETA;
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
-                     one, zero, B2, c->active, &e));
-    e.mode = EPI_DEC_ALPHA;
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
-                     one, zero, B2, c->active, &e));
-    HIPCHK(hipMemsetAsync(c->nactive, 0, sizeof(int), c->stream));
-    dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
-        B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
-        c->active, c->iters, c->status, c->nactive);
-    HIPCHK(hipGetLastError());
-    c->launches++;
+  // Iterations are launched in rounds without waiting: the first round as many
+  // as the previous call needed, then two at a time; the host reads the
+  // active count once per round.  Iterations after a chain has stopped cost
+  // only the early exit of its work-groups (every kernel tests the active
+  // flag), so the results do not depend on the round lengths.
+  int iter = 1, round = std::max(2, std::min(c->dec_hint, itermax + 1));
+  while (iter <= itermax + 1) {
+    const int last = std::min(itermax + 1, iter + round - 1);
+    for (; iter <= last; ++iter) {
+      c->dec_rounds = iter;
+      InvArgs ia = inv_args(c);
+      ia.base = c->p_eps.d;
+      ia.out = c->p_Yinv.d;
+      ia.active = c->active;
+      ia.status = c->status;
+      CHK(run_inverse(c, ia, B2));
+      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
+                       one, zero, B2, c->active));
+      EpiArgs e;
+      memset(&e, 0, sizeof(e));
+      e.n = n;
+      e.eps = c->p_eps.d;
+      e.x = c->p_x.d;
+      e.abn = c->p_AB[cur ^ 1].d;
+      e.norm = c->dnorm;
+      e.tiles = tiles;
+      e.mode = EPI_DEC_BETA;
+      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
+                       one, zero, B2, c->active, &e));
+      e.mode = EPI_DEC_ALPHA;
+      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
+                       one, zero, B2, c->active, &e));
+      dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
+          B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
+          c->active, c->iters, c->status, c->nactive + iter);
+      HIPCHK(hipGetLastError());
+      c->launches++;
+      cur ^= 1;
+    }
     int nact = 0;
-    HIPCHK(hipMemcpyAsync(&nact, c->nactive, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&nact, c->nactive + (iter - 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (nact == 0) break;
-    cur ^= 1;
+    round = 2;
   }
+  // the next call starts with one round of the iterations this one needed:
+  // the first iteration after which no chain was left
+  {
+    int cnt[66] = {0};
+    const int nread = std::min(iter, 66);
+    HIPCHK(hipMemcpyAsync(cnt, c->nactive, sizeof(int) * nread, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int used = iter - 1;
+    for (int j = 1; j < nread; ++j)
+      if (cnt[j] == 0) {
+        used = j;
+        break;
+      }
+    c->dec_hint = used;
+  }
+  HIPCHK(hipMemcpyAsync(c->active, c->active0, B2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
   // g_s = x^-1 for every chain that was started
-  HIPCHK(hipMemcpyAsync(c->active, act.data(), B2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
   InvArgs ia = inv_args(c);
   ia.base = c->p_x.d;
   ia.out = c->p_Yinv.d;
@@ -783,6 +810,7 @@ static int run_sweeps(skkr_ctx* c, int nb) {
     it = c->sweep_graphs.emplace(key, ge).first;
   }
   HIPCHK(hipGraphLaunch(it->second, c->stream));
+  c->launches++;  // one replay counts as one launch
   return SKKR_OK;
 }
 

@@ -359,6 +359,7 @@ def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
     k, w = S.uniform_kmesh(6)
     with gpu.Context.for_system(sy) as ctx:
         ctx.set_tinv(*sy.tinv(E))
+        ctx.set_option("graphs", 0)  # graphs are the default: the eager launches first
         base = ctx.tau_bz(E, k, w)
         ctx.set_option("graphs", 1)
         for rep in range(3):
