@@ -99,6 +99,9 @@ typedef struct {
    * were still iterating (n <= 128) */
   long long inv_launches;
   double inv_ms, inv_flops, inv_bytes;
+  /* batches whose blocked inversions flagged multiplier growth and ran again
+   * with the row-wise kernel that pivots over all rows (gjinv.f:13-22) */
+  int redo_batches;
 } skkr_stats;
 
 /* Context: one per host thread / rank; calls on a context are serialised on

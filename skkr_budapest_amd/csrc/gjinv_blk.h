@@ -20,9 +20,18 @@
 // decimation blocks) have a positive-definite anti-Hermitian part >= Im E,
 // so every diagonal block stays invertible without row exchanges between
 // blocks; on the configurations this matches LAPACK's pivoted inverse to
-// ~1e-13 relative.
+// ~1e-13 relative.  For any other matrix the kernel checks its multipliers:
+// an element of W = -M[O,K] D^-1 above GJ_GROWTH_LIMIT (a diagonal block that
+// is nearly singular or small against the rest of its column) or a zero pivot
+// of a diagonal block sets GJ_STATUS_REDO in the matrix's status, and the host
+// inverts that matrix again with the row-wise kernel, which pivots over all
+// rows as gjinv.f does (and alone decides that a matrix is singular).
 #pragma once
 #include "kernels_f64.h"
+
+// status bit of a matrix whose block elimination grew too large (see above)
+constexpr int GJ_STATUS_REDO = 16;
+constexpr double GJ_GROWTH_LIMIT = 1048576.0;  // 2^20
 
 // 16x16 Gauss-Jordan inversion of a diagonal block held by one wave in the
 // v_mfma_f64_16x16x4 result layout (lane l, element q: row l/16 + 4q, column
@@ -170,6 +179,7 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
   __shared__ double2 Dinv[16 * DP]; // [col][row]
   __shared__ int sperm[16], spinv[16];  // pivot rows of the diagonal block
   __shared__ int s_sing;
+  __shared__ int s_redo;
   __shared__ double2 diag[N];
 
   const int b = blockIdx.x;
@@ -208,7 +218,11 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
           mim[i][j][q] = v.y;
         }
   }
-  if (tid == 0) s_sing = 0;
+  if (tid == 0) {
+    s_sing = 0;
+    s_redo = 0;
+  }
+  bool grown = false;  // a multiplier this lane formed exceeded the limit (or was NaN)
 
   // prologue: block column 0 -> pan; the owner of tile (0,0) inverts D_0
   {
@@ -268,8 +282,10 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
         aim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, d.x, aim, 0, 0, 0);
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q) {
         Wb[l16 * NP + 16 * rho + l4 + 4 * q] = make_double2(-are[q], -aim[q]);
+        grown = grown || !(fabs(are[q]) <= GJ_GROWTH_LIMIT && fabs(aim[q]) <= GJ_GROWTH_LIMIT);
+      }
     }
 #pragma unroll 1
     for (int gam = w; gam < T; gam += NW) {
@@ -383,7 +399,12 @@ __global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel
     __syncthreads();  // the next pan, Dinv and this step's W, X, R reads are done
   }
 
-  if (s_sing && args.status && tid == 0) atomicOr(&args.status[b], 1);
+  if (grown) s_redo = 1;
+  __syncthreads();
+  // a zero pivot of a diagonal block says nothing about the matrix (its rows
+  // never moved between blocks): flagged for the pivoting second pass too
+  if (s_sing && args.status && tid == 0) atomicOr(&args.status[b], 1 | GJ_STATUS_REDO);
+  if (s_redo && args.status && tid == 0) atomicOr(&args.status[b], GJ_STATUS_REDO);
   if (args.mode == 0) {
     double2* __restrict__ out = args.out[b];
 #pragma unroll

@@ -121,6 +121,7 @@ struct skkr_ctx {
   double inv_ms = 0.0, inv_flops = 0.0, inv_bytes = 0.0;  // inversion launches
   long long inv_launches = 0;
   int dec_rounds = 0;  // iteration rounds of the last run_decimation
+  int redo_batches = 0;  // batches of the last tau_bz call run again with full pivoting
   int inv_kernel = 0;  // 0 blocked MFMA Gauss-Jordan, 1 row-wise register Gauss-Jordan
   int gemm_3m = 0;     // 128-wide tiles: 1 = three-product (Gauss) complex GEMM
   int gemm_tile_force = 0;  // 0 = by shape, else 32 | 64 | 96 | 128
@@ -1122,6 +1123,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   c->evclass.clear();
   c->launches = 0;
   c->dom_flops_exec = 0;
+  c->redo_batches = 0;
   // mixed precision (SKKR_MIXED_BF16): split-BF16 products in the decimation
   // and the sweeps; the decimation stays FP64 for Im E below the threshold
   int s_dec = 0, s_sw = 0;
@@ -1175,6 +1177,41 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     {
       SplitScope ss(c, s_sw);
       CHK(run_sweeps(c, nb));
+    }
+    // A blocked inversion whose multipliers grew (GJ_STATUS_REDO: a diagonal
+    // block singular or small against its column) runs the batch's
+    // decimation and sweeps again with the row-wise kernel, which pivots
+    // over all rows.
+    {
+      HIPCHK(hipMemcpyAsync(h_st.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(h_sw.data(), c->status_sw, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      bool redo = false;
+      for (int b = 0; b < nb; ++b)
+        redo = redo || ((h_st[b] | h_st[B + b] | h_sw[b]) & GJ_STATUS_REDO);
+      if (redo) {
+        ++c->redo_batches;
+        struct InvKernel {
+          skkr_ctx* c;
+          int saved;
+          ~InvKernel() { c->inv_kernel = saved; }
+        } keep{c, c->inv_kernel};
+        c->inv_kernel = 1;
+        HIPCHK(hipMemsetAsync(c->status_sw, 0, B * sizeof(int), c->stream));
+        if (!reuse) {
+          HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+          HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+          HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+          SplitScope sd(c, s_dec);
+          CHK(run_decimation(c, true, true, c->str ? &c->p_mt : nullptr));
+          if (c->cpa_mode == 1)
+            HIPCHK(hipMemcpyAsync(c->Y1s, c->Ycur, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+          HIPCHK(hipMemcpyAsync(c->Ycur, c->Y1s, sizeof(z) * c->nn * nb, hipMemcpyDeviceToDevice, c->stream));
+        }
+        SplitScope ss(c, s_sw);
+        CHK(sweep_launches(c, nb));  // eager: the captured graph holds the blocked kernel
+      }
     }
     if (c->pairs_on) CHK(pair_sums(c, nb));
     hipEventRecord(ev[3], c->stream);
@@ -1309,6 +1346,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     st->inv_ms = c->inv_ms;
     st->inv_flops = c->inv_flops;
     st->inv_bytes = c->inv_bytes;
+    st->redo_batches = c->redo_batches;
     snprintf(st->dom_name, sizeof(st->dom_name), "%s",
              (s_dec > 0 && s_sw > 0) ? "zgemm_split_kernel" : (s_sw > 0 ? "zgemm_f64+split" : "zgemm_f64_kernel"));
   }
@@ -1499,8 +1537,9 @@ int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
   if (!c || n <= 0 || n > 256 || nb <= 0 || !a) return SKKR_ERR_ARG;
   if (n > 128 && n != c->n) return SKKR_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->cfg.device));
-  TmpMats m;
+  TmpMats m, orig;
   CHK(m.init(nb, (size_t)n * n, a, c->stream));
+  CHK(orig.init(nb, (size_t)n * n, a, c->stream));  // input kept for a pivoted second pass
   TmpBuf sbuf;
   if (hipMalloc(&sbuf.d, sizeof(int) * nb) != hipSuccess) return SKKR_ERR_NOMEM;
   int* dst = (int*)sbuf.d;
@@ -1515,6 +1554,32 @@ int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
   if (n > 128) CHK(ensure_batch(c, std::max(c->B, (nb + 1) / 2)));
   int r = run_inverse(c, ia, nb);
   std::vector<int> hs(nb);
+  if (r == SKKR_OK && (c->inv_kernel == 0 || c->inv_kernel == 2)) {
+    // matrices whose blocked elimination flagged multiplier growth: inverted
+    // again from the input with the row-wise kernel (pivots over all rows)
+    HIPCHK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int> again(nb);
+    bool any = false;
+    for (int j = 0; j < nb; ++j) {
+      again[j] = (hs[j] & GJ_STATUS_REDO) ? 1 : 0;
+      any = any || again[j];
+    }
+    if (any) {
+      TmpBuf abuf;
+      if (hipMalloc(&abuf.d, sizeof(int) * nb) != hipSuccess) return SKKR_ERR_NOMEM;
+      HIPCHK(hipMemcpy(abuf.d, again.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+      for (int j = 0; j < nb; ++j) hs[j] = again[j] ? 0 : hs[j];
+      HIPCHK(hipMemcpy(dst, hs.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+      InvArgs ib = ia;
+      ib.base = orig.p;
+      ib.active = (int*)abuf.d;
+      const int saved = c->inv_kernel;
+      c->inv_kernel = 1;
+      r = run_inverse(c, ib, nb);
+      c->inv_kernel = saved;
+    }
+  }
   if (r == SKKR_OK) {
     HIPCHK(hipMemcpyAsync(a, m.d, sizeof(z) * n * n * nb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(hs.data(), dst, sizeof(int) * nb, hipMemcpyDeviceToHost, c->stream));
@@ -1564,6 +1629,24 @@ static int dec_host(skkr_ctx* c, int nb, const double* ml, const double* m0, con
   HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
   HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
   CHK(run_decimation(c, add_tinv, sigma_out != nullptr));
+  {
+    // growth flagged in a blocked inversion: the chains again, pivoting over all rows
+    std::vector<int> hs0(2 * B);
+    HIPCHK(hipMemcpyAsync(hs0.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    bool redo = false;
+    for (int j = 0; j < nb; ++j) redo = redo || (hs0[off + j] & GJ_STATUS_REDO);
+    if (redo) {
+      const int saved = c->inv_kernel;
+      c->inv_kernel = 1;
+      HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+      HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+      const int r = run_decimation(c, add_tinv, sigma_out != nullptr);
+      c->inv_kernel = saved;
+      CHK(r);
+    }
+  }
   if (x_out)
     HIPCHK(hipMemcpyAsync(x_out, c->Yinv + off * nn, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   if (sigma_out) {
@@ -1600,6 +1683,27 @@ int skkr_dugo_batch(skkr_ctx* c, char side, int nb, const double* m00, const dou
   return dec_host(c, nb, m01, m00, m10, true, 'R', nullptr, xy, iters, status);
 }
 
+// The sweeps of nb k-points, again with the row-wise (fully pivoting)
+// inversion when a blocked inversion flagged multiplier growth.  The forward
+// sweep overwrites Y_1 (Ycur) with the later Y_p: the host's y is copied in
+// again before the second pass.
+static int sweeps_checked(skkr_ctx* c, int nb, const double* y_host) {
+  CHK(run_sweeps(c, nb));
+  std::vector<int> hs(nb);
+  HIPCHK(hipMemcpyAsync(hs.data(), c->status_sw, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  bool redo = false;
+  for (int j = 0; j < nb; ++j) redo = redo || (hs[j] & GJ_STATUS_REDO);
+  if (!redo) return SKKR_OK;
+  const int saved = c->inv_kernel;
+  c->inv_kernel = 1;
+  HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
+  HIPCHK(hipMemcpyAsync(c->Ycur, y_host, sizeof(z) * c->nn * nb, hipMemcpyHostToDevice, c->stream));
+  const int r = sweep_launches(c, nb);
+  c->inv_kernel = saved;
+  return r;
+}
+
 int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
                      const double* m10, const double* x, const double* y, double w,
                      double* tau, int* status) {
@@ -1616,7 +1720,7 @@ int skkr_tau2d_batch(skkr_ctx* c, int nb, const double* m00, const double* m01,
   HIPCHK(hipMemcpyAsync(c->Xc, x, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->Ycur, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
-  CHK(run_sweeps(c, nb));
+  CHK(sweeps_checked(c, nb, y));
   long long ne = (long long)tsz * nb;
   scale_copy_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(ne, c->tauk, w, c->tauk);
   HIPCHK(hipGetLastError());
@@ -1658,7 +1762,7 @@ int skkr_tau2d_layered_batch(skkr_ctx* c, int nb, const double* m00, const doubl
   HIPCHK(hipMemcpyAsync(c->Ycur, y, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->status_sw, 0, nb * sizeof(int), c->stream));
   c->layered = true;
-  const int rs = run_sweeps(c, nb);
+  const int rs = sweeps_checked(c, nb, y);
   c->layered = false;
   CHK(rs);
   long long ne = (long long)tsz * nb;
@@ -1827,6 +1931,10 @@ int skkr_str_create(skkr_str** out, const skkr_str_config* cfg, const double* cv
     r = skkr_create(&sc->invS, &ic);
     if (r != SKKR_OK) return fail(r);
   }
+  // 1 - alpha G has no definite anti-Hermitian part: its inversions pivot over
+  // all rows (the row-wise kernel), within the halves of a 2 x 2 split
+  sc->inv->inv_kernel = 1;
+  if (sc->invS) sc->invS->inv_kernel = 1;
   sc->stream = sc->inv->stream;
   // lattice and the stack of layers (gstore.f:99-122)
   sc->lat = kkrh::make_lattice(cfg->a1, cfg->a2, 20000);

@@ -30,6 +30,7 @@ module skkr_gpu
      real(c_double) :: dom_bytes, dom_flops_exec
      integer(c_long_long) :: inv_launches
      real(c_double) :: inv_ms, inv_flops, inv_bytes
+     integer(c_int) :: redo_batches
   end type skkr_stats
 
   ! layer structure-constant producer (gstore.f with bulkgeo)

@@ -63,7 +63,8 @@ class Stats(ctypes.Structure):
                 ("dom_flops", ctypes.c_double), ("dom_name", ctypes.c_char * 32),
                 ("dom_bytes", ctypes.c_double), ("dom_flops_exec", ctypes.c_double),
                 ("inv_launches", ctypes.c_longlong), ("inv_ms", ctypes.c_double),
-                ("inv_flops", ctypes.c_double), ("inv_bytes", ctypes.c_double)]
+                ("inv_flops", ctypes.c_double), ("inv_bytes", ctypes.c_double),
+                ("redo_batches", ctypes.c_int)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_}

@@ -201,6 +201,17 @@ def _herm_site(rng, lk):
     return h + h.conj().T + np.diag(rng.uniform(-1.0, 1.0, size=lk))
 
 
+def _dense_site(rng, lk):
+    """Dense random complex site block with every (spin) sub-block filled,
+    diagonal scaled to twice the off-diagonal row sum (|d_ii| = 2 sum_j |a_ij|),
+    with phases in the upper half plane: the literal law of the
+    relativistic configurations' t^-1."""
+    a = (rng.standard_normal((lk, lk)) + 1j * rng.standard_normal((lk, lk))) / np.sqrt(2.0)
+    np.fill_diagonal(a, 0.0)
+    d = 2.0 * np.abs(a).sum(1) * np.exp(1j * rng.uniform(0.0, np.pi, size=lk))
+    return a + np.diag(d)
+
+
 def _gen_t0(cfg: Config, rng, nsites: int) -> np.ndarray:
     lk = cfg.lk
     t0 = np.zeros((nsites, lk, lk), dtype=np.complex128)
@@ -210,6 +221,8 @@ def _gen_t0(cfg: Config, rng, nsites: int) -> np.ndarray:
             t0[a] = np.diag(d)
         elif cfg.tmodel == "herm":
             t0[a] = _herm_site(rng, lk)
+        elif cfg.tmodel == "dense":
+            t0[a] = _dense_site(rng, lk)
         elif cfg.tmodel == "bdg":
             h = lk // 2
             he = _herm_site(rng, h)
@@ -254,6 +267,9 @@ class System:
         "diag" (nonrelativistic, config 0): E-independent diagonal blocks.
         "herm"/"bdg": t^-1(E) = E*I + T0 with Hermitian T0, so that
         t^-1 - G(k) = E - H(k) and Im E alone sets the absorption.
+        "dense": t^-1(E) = E*I + T0 with T0 dense, random and diagonally
+        dominant (no Hermitian structure: t^-1 - G(k) has no definite
+        anti-Hermitian part).
         """
         if self.cfg.tmodel == "diag":
             return self.t0_int.copy(), self.t0_left.copy(), self.t0_right.copy()
