@@ -64,7 +64,8 @@ struct skkr_ctx {
   skkr_config cfg;
   int n = 0, nn = 0, ns = 0, nR = 0;
   hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
+  std::vector<void*> allocs;        // freed by skkr_destroy
+  std::vector<void*> batch_allocs;  // sized by the batch: freed when the batch grows
   // structure and t^-1
   double* dR = nullptr;
   z* dGRi = nullptr;  // intra
@@ -192,15 +193,39 @@ static int dalloc_t(skkr_ctx* c, T** p, size_t count) {
   return dalloc(c, (void**)p, count * sizeof(T));
 }
 
-static int make_ptrs(skkr_ctx* c, PtrArr* pa, const std::vector<z*>& v) {
-  CHK(dalloc_t(c, &pa->d, v.size()));
+// buffers sized by the batch (ensure_batch and the per-batch extras)
+static int dalloc_b(skkr_ctx* c, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (hipMalloc(p, bytes) != hipSuccess) return SKKR_ERR_NOMEM;
+  c->batch_allocs.push_back(*p);
+  return SKKR_OK;
+}
+template <class T>
+static int dalloc_bt(skkr_ctx* c, T** p, size_t count) {
+  return dalloc_b(c, (void**)p, count * sizeof(T));
+}
+
+static int make_ptrs(skkr_ctx* c, PtrArr* pa, const std::vector<z*>& v, bool batch = false) {
+  if (batch)
+    CHK(dalloc_bt(c, &pa->d, v.size()));
+  else
+    CHK(dalloc_t(c, &pa->d, v.size()));
   pa->n = (int)v.size();
   HIPCHK(hipMemcpy(pa->d, v.data(), v.size() * sizeof(z*), hipMemcpyHostToDevice));
   return SKKR_OK;
 }
 
+// Frees every buffer sized by the batch (before a larger batch is allocated);
+// the capacities of the per-batch extras go back to zero, so they are
+// allocated again at the new size when next used.
 static int free_batch(skkr_ctx* c) {
-  // buffers live until destroy; a new batch size reallocates everything
+  if (c->batch_allocs.empty()) return SKKR_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (void* p : c->batch_allocs) hipFree(p);
+  c->batch_allocs.clear();
+  c->B = 0;
+  c->invB = c->ws_cap = c->pl_cap = c->mt_cap = c->phk_cap = c->y1_cap = 0;
+  c->Asc = c->Csc = c->Dsc = nullptr;
   return SKKR_OK;
 }
 
@@ -216,47 +241,47 @@ static int ensure_batch(skkr_ctx* c, int B) {
   const int n = c->n;
   const size_t nn = c->nn, P = c->cfg.n_pl, lk = c->cfg.lk, ns = c->ns;
   const size_t B2 = 2 * (size_t)B;
-  CHK(dalloc_t(c, &c->dk, 2 * (size_t)B));
-  CHK(dalloc_t(c, &c->dw, (size_t)B));
-  CHK(dalloc_t(c, &c->M00, B * nn));
-  CHK(dalloc_t(c, &c->M01, B * nn));
-  CHK(dalloc_t(c, &c->M10, B * nn));
-  CHK(dalloc_t(c, &c->eps, B2 * nn));
-  CHK(dalloc_t(c, &c->xs, B2 * nn));
-  CHK(dalloc_t(c, &c->AB, 2 * B2 * 2 * nn));
+  CHK(dalloc_bt(c, &c->dk, 2 * (size_t)B));
+  CHK(dalloc_bt(c, &c->dw, (size_t)B));
+  CHK(dalloc_bt(c, &c->M00, B * nn));
+  CHK(dalloc_bt(c, &c->M01, B * nn));
+  CHK(dalloc_bt(c, &c->M10, B * nn));
+  CHK(dalloc_bt(c, &c->eps, B2 * nn));
+  CHK(dalloc_bt(c, &c->xs, B2 * nn));
+  CHK(dalloc_bt(c, &c->AB, 2 * B2 * 2 * nn));
   // |alpha'|^2 partials: one per work-group of the 2n x n product, at most
   // the count of the smallest (32 x 32) tile
-  CHK(dalloc_t(c, &c->dnorm, (size_t)B2 * ((2 * n + 31) / 32) * ((n + 31) / 32)));
-  CHK(dalloc_t(c, &c->Yinv, B2 * nn));
-  CHK(dalloc_t(c, &c->AYBY, B2 * 2 * nn));
-  CHK(dalloc_t(c, &c->active, B2));
-  CHK(dalloc_t(c, &c->iters, B2));
-  CHK(dalloc_t(c, &c->status, B2));
-  CHK(dalloc_t(c, &c->itermax, B2));
+  CHK(dalloc_bt(c, &c->dnorm, (size_t)B2 * ((2 * n + 31) / 32) * ((n + 31) / 32)));
+  CHK(dalloc_bt(c, &c->Yinv, B2 * nn));
+  CHK(dalloc_bt(c, &c->AYBY, B2 * 2 * nn));
+  CHK(dalloc_bt(c, &c->active, B2));
+  CHK(dalloc_bt(c, &c->iters, B2));
+  CHK(dalloc_bt(c, &c->status, B2));
+  CHK(dalloc_bt(c, &c->itermax, B2));
   // still-active chain count after each decimation iteration (index = iteration)
-  CHK(dalloc_t(c, &c->nactive, (size_t)std::max(c->cfg.itermax_l, c->cfg.itermax_r) + 2));
-  CHK(dalloc_t(c, &c->active0, B2));
-  CHK(dalloc_t(c, &c->status_sw, (size_t)B));
-  CHK(dalloc_t(c, &c->Gs, P * B * nn));
-  CHK(dalloc_t(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
-  CHK(dalloc_t(c, &c->Ycur, B * nn));
-  CHK(dalloc_t(c, &c->W, B * nn));
-  CHK(dalloc_t(c, &c->Zb, B * nn));
-  CHK(dalloc_t(c, &c->Xc, B * nn));
-  CHK(dalloc_t(c, &c->tauk, (size_t)B * P * ns * lk * lk / (ns ? 1 : 1)));
-  CHK(dalloc_t(c, &c->trk, (size_t)B * P * ns));
+  CHK(dalloc_bt(c, &c->nactive, (size_t)std::max(c->cfg.itermax_l, c->cfg.itermax_r) + 2));
+  CHK(dalloc_bt(c, &c->active0, B2));
+  CHK(dalloc_bt(c, &c->status_sw, (size_t)B));
+  CHK(dalloc_bt(c, &c->Gs, P * B * nn));
+  CHK(dalloc_bt(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
+  CHK(dalloc_bt(c, &c->Ycur, B * nn));
+  CHK(dalloc_bt(c, &c->W, B * nn));
+  CHK(dalloc_bt(c, &c->Zb, B * nn));
+  CHK(dalloc_bt(c, &c->Xc, B * nn));
+  CHK(dalloc_bt(c, &c->tauk, (size_t)B * P * ns * lk * lk / (ns ? 1 : 1)));
+  CHK(dalloc_bt(c, &c->trk, (size_t)B * P * ns));
   if (n > 128 && c->invB < B) {
     const size_t h1 = 128, h2 = n - 128;
-    CHK(dalloc_t(c, &c->Asc, B2 * nn));
-    CHK(dalloc_t(c, &c->Csc, B2 * h1 * h2));
-    CHK(dalloc_t(c, &c->Dsc, B2 * h1 * h2));
-    for (int v = 0; v < 4; ++v) CHK(dalloc_t(c, &c->dv_out[v], B2));
+    CHK(dalloc_bt(c, &c->Asc, B2 * nn));
+    CHK(dalloc_bt(c, &c->Csc, B2 * h1 * h2));
+    CHK(dalloc_bt(c, &c->Dsc, B2 * h1 * h2));
+    for (int v = 0; v < 4; ++v) CHK(dalloc_bt(c, &c->dv_out[v], B2));
   }
   std::vector<z*> v(B2);
   auto fill = [&](PtrArr* pa, auto fn, size_t cnt) -> int {
     std::vector<z*> t(cnt);
     for (size_t j = 0; j < cnt; ++j) t[j] = fn(j);
-    return make_ptrs(c, pa, t);
+    return make_ptrs(c, pa, t, true);
   };
   const size_t Bs = B;
   // decimation chain j: left lead for j < B (ml = M10, mr = M01, tinv = left),
@@ -314,14 +339,14 @@ static int ensure_inverse(skkr_ctx* c, int B) {
   const int n = c->n;
   if (n <= 128 || c->invB >= B) return SKKR_OK;
   const size_t nn = c->nn, B2 = 2 * (size_t)B, h1 = 128, h2 = n - 128;
-  CHK(dalloc_t(c, &c->Asc, B2 * nn));
-  CHK(dalloc_t(c, &c->Csc, B2 * h1 * h2));
-  CHK(dalloc_t(c, &c->Dsc, B2 * h1 * h2));
-  for (int v = 0; v < 4; ++v) CHK(dalloc_t(c, &c->dv_out[v], B2));
+  CHK(dalloc_bt(c, &c->Asc, B2 * nn));
+  CHK(dalloc_bt(c, &c->Csc, B2 * h1 * h2));
+  CHK(dalloc_bt(c, &c->Dsc, B2 * h1 * h2));
+  for (int v = 0; v < 4; ++v) CHK(dalloc_bt(c, &c->dv_out[v], B2));
   auto fill = [&](PtrArr* pa, auto fn, size_t cnt) -> int {
     std::vector<z*> t(cnt);
     for (size_t j = 0; j < cnt; ++j) t[j] = fn(j);
-    return make_ptrs(c, pa, t);
+    return make_ptrs(c, pa, t, true);
   };
   CHK(fill(&c->p_Asc, [&](size_t j) { return c->Asc + j * nn; }, B2));
   CHK(fill(&c->p_A21, [&](size_t j) { return c->Asc + j * nn + h1; }, B2));
@@ -893,6 +918,7 @@ void skkr_destroy(skkr_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graphs(c);
   for (void* p : c->allocs) hipFree(p);
+  for (void* p : c->batch_allocs) hipFree(p);
   if (c->stream) hipStreamDestroy(c->stream);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   for (hipEvent_t e : c->phase_ev)
@@ -963,12 +989,12 @@ static int ensure_ws(skkr_ctx* c) {
   const int P = c->cfg.n_pl;
   if (c->ws_cap >= c->B || P < 2) return SKKR_OK;
   const size_t nn = c->nn, Bs = c->B;
-  CHK(dalloc_t(c, &c->Ws, (size_t)(P - 1) * Bs * nn));
+  CHK(dalloc_bt(c, &c->Ws, (size_t)(P - 1) * Bs * nn));
   c->p_Ws.assign(P - 1, PtrArr());
   for (int p = 0; p < P - 1; ++p) {
     std::vector<z*> v(Bs);
     for (size_t j = 0; j < Bs; ++j) v[j] = c->Ws + ((size_t)p * Bs + j) * nn;
-    CHK(make_ptrs(c, &c->p_Ws[p], v));
+    CHK(make_ptrs(c, &c->p_Ws[p], v, true));
   }
   c->ws_cap = c->B;
   return SKKR_OK;
@@ -979,20 +1005,20 @@ static int ensure_layered(skkr_ctx* c) {
   const int P = c->cfg.n_pl;
   if (c->pl_cap >= c->B) return SKKR_OK;
   const size_t nn = c->nn, Bs = c->B;
-  CHK(dalloc_t(c, &c->M00pl, (size_t)P * Bs * nn));
-  CHK(dalloc_t(c, &c->M01pl, (size_t)P * Bs * nn));
-  CHK(dalloc_t(c, &c->M10pl, (size_t)P * Bs * nn));
+  CHK(dalloc_bt(c, &c->M00pl, (size_t)P * Bs * nn));
+  CHK(dalloc_bt(c, &c->M01pl, (size_t)P * Bs * nn));
+  CHK(dalloc_bt(c, &c->M10pl, (size_t)P * Bs * nn));
   c->p_M00pl.assign(P, PtrArr());
   c->p_M01pl.assign(P, PtrArr());
   c->p_M10pl.assign(P, PtrArr());
   for (int p = 0; p < P; ++p) {
     std::vector<z*> v(Bs);
     for (size_t j = 0; j < Bs; ++j) v[j] = c->M00pl + ((size_t)p * Bs + j) * nn;
-    CHK(make_ptrs(c, &c->p_M00pl[p], v));
+    CHK(make_ptrs(c, &c->p_M00pl[p], v, true));
     for (size_t j = 0; j < Bs; ++j) v[j] = c->M01pl + ((size_t)p * Bs + j) * nn;
-    CHK(make_ptrs(c, &c->p_M01pl[p], v));
+    CHK(make_ptrs(c, &c->p_M01pl[p], v, true));
     for (size_t j = 0; j < Bs; ++j) v[j] = c->M10pl + ((size_t)p * Bs + j) * nn;
-    CHK(make_ptrs(c, &c->p_M10pl[p], v));
+    CHK(make_ptrs(c, &c->p_M10pl[p], v, true));
   }
   c->pl_cap = c->B;
   return SKKR_OK;
@@ -1057,11 +1083,11 @@ static int pair_sums(skkr_ctx* c, int nb) {
 static int ensure_mt(skkr_ctx* c) {
   if (c->mt_cap >= c->B) return SKKR_OK;
   const size_t nn = c->nn, Bs = c->B;
-  CHK(dalloc_t(c, &c->MtL, Bs * nn));
-  CHK(dalloc_t(c, &c->MtR, Bs * nn));
+  CHK(dalloc_bt(c, &c->MtL, Bs * nn));
+  CHK(dalloc_bt(c, &c->MtR, Bs * nn));
   std::vector<z*> v(2 * Bs);
   for (size_t j = 0; j < 2 * Bs; ++j) v[j] = j < Bs ? c->MtL + j * nn : c->MtR + (j - Bs) * nn;
-  CHK(make_ptrs(c, &c->p_mt, v));
+  CHK(make_ptrs(c, &c->p_mt, v, true));
   c->mt_cap = c->B;
   return SKKR_OK;
 }
@@ -1224,7 +1250,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     if (phys && taukdiag) {
       const int nl = c->nl;
       if (c->phk_cap < B) {
-        CHK(dalloc_t(c, &c->phk, (size_t)B * sites * nl));
+        CHK(dalloc_bt(c, &c->phk, (size_t)B * sites * nl));
         c->phk_cap = B;
       }
       dim3 grid((unsigned)sites, (unsigned)nb);
@@ -1551,7 +1577,7 @@ int skkr_gjinv_batch(skkr_ctx* c, int n, int nb, double* a, int* status) {
   ia.base = m.p;
   ia.out = m.p;
   ia.status = dst;
-  if (n > 128) CHK(ensure_batch(c, std::max(c->B, (nb + 1) / 2)));
+  if (n > 128) CHK(ensure_inverse(c, std::max(c->invB, (nb + 1) / 2)));
   int r = run_inverse(c, ia, nb);
   std::vector<int> hs(nb);
   if (r == SKKR_OK && (c->inv_kernel == 0 || c->inv_kernel == 2)) {
@@ -2505,7 +2531,7 @@ int skkr_cpa_bz(skkr_ctx* c, double e_re, double e_im, int nk, const double* k, 
   if (resident) {
     CHK(ensure_batch(c, std::max(c->B, nk)));
     if (c->y1_cap < c->B) {
-      CHK(dalloc_t(c, &c->Y1s, (size_t)c->B * c->nn));
+      CHK(dalloc_bt(c, &c->Y1s, (size_t)c->B * c->nn));
       c->y1_cap = c->B;
     }
   }

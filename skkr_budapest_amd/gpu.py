@@ -38,6 +38,11 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch", "skkr_set_point_group")
 
 
+# codes that name a failing k-point or matrix (reported per item in status
+# arrays, as the reference stops there); the others are call errors
+PER_ITEM = (SKKR_OK, 4, 5, 6, 8)
+
+
 class SkkrError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"{what}: {ERRORS.get(code, code)}")
@@ -165,6 +170,13 @@ class Context:
     @staticmethod
     def _check(rc, what):
         if rc != SKKR_OK:
+            raise SkkrError(rc, what)
+
+    @staticmethod
+    def _check_call(rc, what):
+        """Raise on call errors (argument, HIP, memory, unsupported); per-item
+        failures stay in the returned status arrays."""
+        if rc not in PER_ITEM:
             raise SkkrError(rc, what)
 
     def close(self):
@@ -343,8 +355,7 @@ class Context:
         buf = colmajor(_c128(a))
         status = np.zeros(nb, np.int32)
         rc = lib().skkr_gjinv_batch(self._h, n, nb, _p(buf), _p(status))
-        if rc not in (SKKR_OK, 4):
-            self._check(rc, "skkr_gjinv_batch")
+        self._check_call(rc, "skkr_gjinv_batch")
         return from_colmajor(buf), status
 
     def zgemm(self, a, b):
@@ -360,7 +371,8 @@ class Context:
         x = np.zeros_like(bufs[0])
         iters = np.zeros(nb, np.int32)
         status = np.zeros(nb, np.int32)
-        lib().skkr_surfgfd_batch(self._h, nb, *map(_p, bufs), _p(x), _p(iters), _p(status))
+        self._check_call(lib().skkr_surfgfd_batch(self._h, nb, *map(_p, bufs), _p(x), _p(iters), _p(status)),
+                         "skkr_surfgfd_batch")
         return from_colmajor(x), iters, status
 
     def dugo(self, side, m00, m01, m10):
@@ -369,8 +381,8 @@ class Context:
         xy = np.zeros_like(bufs[0])
         iters = np.zeros(nb, np.int32)
         status = np.zeros(nb, np.int32)
-        lib().skkr_dugo_batch(self._h, side.encode(), nb, *map(_p, bufs), _p(xy), _p(iters),
-                              _p(status))
+        self._check_call(lib().skkr_dugo_batch(self._h, side.encode(), nb, *map(_p, bufs), _p(xy),
+                                               _p(iters), _p(status)), "skkr_dugo_batch")
         return from_colmajor(xy), iters, status
 
     def tau2d(self, m00, m01, m10, x, y, w=1.0):
@@ -380,7 +392,8 @@ class Context:
         tau = np.zeros((nb, ns, self.lk, self.lk), np.complex128)
         tb = colmajor(tau)
         status = np.zeros(nb, np.int32)
-        lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
+        self._check_call(lib().skkr_tau2d_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status)),
+                         "skkr_tau2d_batch")
         return from_colmajor(tb), status
 
     def set_point_group(self, dg=None, dg1=None):
@@ -407,8 +420,7 @@ class Context:
         tb = colmajor(tau)
         status = np.zeros(nb, np.int32)
         rc = lib().skkr_tau2d_layered_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
-        if rc == 1:
-            self._check(rc, "skkr_tau2d_layered_batch")
+        self._check_call(rc, "skkr_tau2d_layered_batch")
         return from_colmajor(tb), status
 
 

@@ -14,16 +14,19 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "skkr_budapest_amd", "fortran", "skkr_kloop")
 
 
-def write_input(path, sy, E, k, w):
+def write_input(path, sy, E, k, w, lidx, a):
     c = sy.cfg
+    nl = int(lidx.max()) + 1
     with open(path, "wb") as f:
-        np.array([c.lk, c.s, c.P, len(sy.R), len(w), c.itermax], np.int32).tofile(f)
+        np.array([c.lk, c.s, c.P, len(sy.R), len(w), c.itermax, nl], np.int32).tofile(f)
         np.array([c.errmax, E.real, E.imag], np.float64).tofile(f)
         np.ascontiguousarray(sy.R, np.float64).tofile(f)
-        for a in (sy.G_intra, sy.G_inter, *sy.tinv(E)):
-            S.colmajor(a).tofile(f)
+        for m in (sy.G_intra, sy.G_inter, *sy.tinv(E)):
+            S.colmajor(m).tofile(f)
         np.ascontiguousarray(k, np.float64).tofile(f)
         np.ascontiguousarray(w, np.float64).tofile(f)
+        np.ascontiguousarray(lidx, np.int32).tofile(f)
+        np.ascontiguousarray(a, np.complex128).tofile(f)  # a[site][index] = alpha(index, site)
 
 
 @pytest.mark.skipif(shutil.which("flang") is None, reason="no flang")
@@ -36,20 +39,33 @@ def test_fortran_host_builds_and_links():
 @pytest.mark.gpu
 @pytest.mark.parametrize("nprocs", [1, 3])
 def test_fortran_kloop_matches_oracle(tmp_path, nprocs):
-    cfg = S.small_config(lk=6, s=2, P=4, r2max=4, energies=(0.4 + 0.07j,), mesh=5)
+    """The cpacoord k-loop replaced from Fortran: the screened BZ sum tau and
+    taukdiag(ik, 0:lmax, li), the per-l traces of phystau(tau(k))
+    (cpacoord.f:43,294-309), against the oracle, whose phystau is pinned to
+    the reference's in test_oracle.py."""
+    cfg = S.small_config(lk=9, s=2, P=4, r2max=4, energies=(0.4 + 0.07j,), mesh=5)
     sy = S.make_system(cfg)
     E = cfg.energies[0]
     k, w = S.uniform_kmesh(cfg.mesh)
+    ns, lk, nk = cfg.P * cfg.s, cfg.lk, len(w)
+    lidx = np.array([0, 1, 1, 1, 2, 2, 2, 2, 2], np.int32)  # lm -> l for lmax 2
+    nl = 3
+    rng = np.random.default_rng(5)
+    alpha_l = 0.2 * (rng.normal(size=(ns, nl)) + 1j * rng.normal(size=(ns, nl)))
+    a = alpha_l[:, lidx]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    write_input(fin, sy, E, k, w)
+    write_input(fin, sy, E, k, w, lidx, a)
     subprocess.run([EXE, str(fin), str(fout), str(nprocs)], check=True, timeout=120)
     raw = np.fromfile(fout, np.complex128)
-    ns, lk, nk = cfg.P * cfg.s, cfg.lk, len(w)
     tau = S.from_colmajor(raw[:ns * lk * lk].reshape(ns, lk, lk))
-    tkd = raw[ns * lk * lk:].reshape(ns, nk).T
-    ref = oracle.tau_bz(sy, E, k, w)
+    tkd = raw[ns * lk * lk:].reshape(ns, nl, nk)  # Fortran taukdiag(ik, l, li)
+    ref = oracle.tau_bz(sy, E, k, w, want_tau_k=True)
     assert np.linalg.norm(tau - ref["tau_bz"]) < 1e-11 * np.linalg.norm(ref["tau_bz"])
-    assert np.abs(tkd - ref["trk"]).max() < 1e-11 * np.abs(ref["trk"]).max()
+    tint = sy.tinv(E)[0]
+    tkd_ref = np.array([[oracle.phys_traces(ref["tau_k"][ik, i], tint[i], a[i], lidx, nl)
+                         for ik in range(nk)] for i in range(ns)])  # (site, k, l)
+    got = np.transpose(tkd, (0, 2, 1))
+    assert np.abs(got - tkd_ref).max() < 1e-11 * np.abs(tkd_ref).max()
 
 
 GEXE = os.path.join(ROOT, "skkr_budapest_amd", "fortran", "skkr_gstore")
