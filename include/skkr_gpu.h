@@ -38,8 +38,9 @@ enum {
   SKKR_ERR_SGF_DIVERGED = 5, /* decimation |alpha|_F > 1e10 (surfgf.f:89-94) */
   SKKR_ERR_SGF_ITERMAX = 6,  /* decimation iter > itermax (surfgf.f:100-103) */
   SKKR_ERR_UNSUPPORTED = 7,  /* size/mode not built */
-  SKKR_ERR_STR_NOCONV = 8    /* structure-constant shell sum not converged
+  SKKR_ERR_STR_NOCONV = 8,   /* structure-constant shell sum not converged
                                 (kamstr.f:358-359,388-389, directg.f:128-131) */
+  SKKR_ERR_TOO_LARGE = 9     /* skkr_surfgfd_noalloc: more than its static work space */
 };
 
 /* precision modes (no reference counterpart; the reference is FP64 only) */
@@ -380,6 +381,28 @@ int skkr_tau_bz_pairs(skkr_ctx *ctx, double e_re, double e_im, int nk,
                       const double *k, const double *w, double *tau_bz,
                       double *tau_ij, double *tau_ji, int *iters_k,
                       int *status_k, skkr_stats *stats);
+
+/* ------------------------------------------------------------------------
+ * Decimation that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  The same decimation as
+ * skkr_surfgfd_batch (surfgfd, skkr_kpar_v6/main/surfgf.f:2-108, for nb
+ * independent chains), but every argument is a DEVICE pointer the caller owns:
+ *   ml, m0, mr  nb consecutive n x n complex matrices each (column-major)
+ *   x           out: nb surface Green's functions
+ *   iters       out: nb decimation iteration counts
+ *   status      out: nb SKKR_* codes per chain
+ * errmax, itermax as in surfgfd (itermax <= 64); stream: a hipStream_t (NULL:
+ * the default stream); the call returns when the results are in place.
+ * The chains' work buffers (9 n^2 complex per chain, rounded up to an even
+ * chain count, plus bookkeeping) live in one 64 MiB array in device memory
+ * that the library holds on every GPU it runs on until the process ends:
+ * n <= 128 and nb <= 1024, and larger problems (e.g. more than 28 chains of
+ * n = 128, 358 of n = 36) return SKKR_ERR_TOO_LARGE.  Calls that overlap on
+ * one GPU share that array and race: serialise them. */
+int skkr_surfgfd_noalloc(int device, int n, int nb, const double *ml, const double *m0,
+                         const double *mr, double *x, int *iters, int *status,
+                         double errmax, int itermax, void *stream);
 
 #ifdef __cplusplus
 }

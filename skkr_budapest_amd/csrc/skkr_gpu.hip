@@ -38,6 +38,9 @@
     }                                                                      \
   } while (0)
 
+// work space of skkr_surfgfd_noalloc, per GPU (64 MiB)
+#define SKKR_NOALLOC_SCRATCH_BYTES (64 << 20)
+
 #define CHK(x)                  \
   do {                          \
     int r_ = (x);               \
@@ -864,6 +867,7 @@ const char* skkr_error_string(int code) {
     case SKKR_ERR_SGF_ITERMAX: return "decimation exceeded itermax";
     case SKKR_ERR_UNSUPPORTED: return "unsupported size or mode";
     case SKKR_ERR_STR_NOCONV: return "structure-constant shell sum not converged";
+    case SKKR_ERR_TOO_LARGE: return "problem larger than the static work space";
   }
   return "unknown error";
 }
@@ -1698,6 +1702,177 @@ int skkr_surfgfd_batch(skkr_ctx* c, int nb, const double* ml, const double* m0,
   if (!c || nb <= 0 || !ml || !m0 || !mr || !x) return SKKR_ERR_ARG;
   HIPCHK(hipSetDevice(c->cfg.device));
   return dec_host(c, nb, ml, m0, mr, false, 'L', x, nullptr, iters, status);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------
+// Decimation without allocation (skkr_surfgfd_noalloc): the work buffers of
+// the chains live in one statically allocated device array per GPU.
+// ------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SKKR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDev = 64;
+void* g_scratch[kMaxDev] = {};  // device address of scratch on each GPU
+}  // namespace
+
+// bytes of scratch that nb chains of n x n take (256-byte aligned pieces)
+static size_t noalloc_bytes(int n, int nb, int itermax) {
+  const size_t B2 = 2 * (size_t)((nb + 1) / 2), nn = (size_t)n * n;
+  const size_t tiles = (size_t)((2 * n + 31) / 32) * ((n + 31) / 32);
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  return al(B2 * nn * sizeof(double2)) * 2          // eps, x
+         + al(2 * B2 * 2 * nn * sizeof(double2))    // two [alpha; beta]
+         + al(B2 * nn * sizeof(double2))            // y
+         + al(B2 * 2 * nn * sizeof(double2))        // [ay; by]
+         + al(B2 * tiles * sizeof(double))          // |alpha'|^2 partials
+         + 2 * al(B2 * sizeof(int))                 // active, active0
+         + al(((size_t)itermax + 2) * sizeof(int))  // active count per iteration
+         + al(2 * B2 * sizeof(int))                 // iterations, status
+         + 13 * al(B2 * sizeof(double2*));          // pointer arrays
+}
+
+// device address of this GPU's scratch (looked up on the first call per device)
+static int scratch_ptr(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDev) return SKKR_ERR_ARG;
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch[dev];
+  return SKKR_OK;
+}
+
+// out[j] = base + j * stride (j < count), and base for count <= j < total
+__global__ void fill_ptrs_kernel(double2** out, double2* base, long long stride, int count, int total) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < total) out[j] = base + (j < count ? j : 0) * stride;
+}
+__global__ void fill_int_kernel(int* out, int value, int count, int total) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < total) out[j] = (j < count) ? value : 0;
+}
+
+extern "C" {
+
+int skkr_surfgfd_noalloc(int device, int n, int nb, const double* ml, const double* m0, const double* mr,
+                         double* x, int* iters, int* status, double errmax, int itermax, void* stream) {
+  if (n <= 0 || n > 128 || nb <= 0 || !ml || !m0 || !mr || !x || !iters || !status || errmax <= 0 ||
+      itermax < 1 || itermax > 64)
+    return SKKR_ERR_ARG;
+  const size_t need = noalloc_bytes(n, nb, itermax);
+  if (need > (size_t)SKKR_NOALLOC_SCRATCH_BYTES || nb > 1024) {
+    fprintf(stderr,
+            "skkr_surfgfd_noalloc: %d chains of %d x %d need %zu bytes of work space; the static scratch "
+            "holds %d bytes and at most 1024 chains\n",
+            nb, n, n, need, (int)SKKR_NOALLOC_SCRATCH_BYTES);
+    return SKKR_ERR_TOO_LARGE;
+  }
+  HIPCHK(hipSetDevice(device));
+  unsigned char* base = nullptr;
+  CHK(scratch_ptr(device, &base));
+  // a context on the host stack whose buffers are views of the scratch and
+  // of the caller's arrays; it owns no device memory (allocs stay empty)
+  skkr_ctx v;
+  v.cfg.lk = n;
+  v.cfg.sites_per_pl = 1;
+  v.cfg.n_pl = 1;
+  v.cfg.errmax = errmax;
+  v.cfg.itermax_l = v.cfg.itermax_r = itermax;
+  v.cfg.precision = SKKR_FP64;
+  v.cfg.device = device;
+  v.n = n;
+  v.nn = n * n;
+  v.ns = 1;
+  v.stream = (hipStream_t)stream;
+  v.graphs = 0;
+  const int B = (nb + 1) / 2, B2 = 2 * B;
+  v.B = B;
+  const size_t nn = (size_t)n * n;
+  const size_t tiles = (size_t)((2 * n + 31) / 32) * ((n + 31) / 32);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  z* eps = (z*)take(B2 * nn * sizeof(z));
+  z* xs = (z*)take(B2 * nn * sizeof(z));
+  z* AB = (z*)take(2 * (size_t)B2 * 2 * nn * sizeof(z));
+  z* Yinv = (z*)take(B2 * nn * sizeof(z));
+  z* AYBY = (z*)take(B2 * 2 * nn * sizeof(z));
+  v.dnorm = (double*)take(B2 * tiles * sizeof(double));
+  v.active = (int*)take(B2 * sizeof(int));
+  v.active0 = (int*)take(B2 * sizeof(int));
+  v.nactive = (int*)take(((size_t)itermax + 2) * sizeof(int));
+  // iterations and status of the chains (those beyond nb are padding)
+  int* itst = (int*)take(2 * (size_t)B2 * sizeof(int));
+  v.iters = itst;
+  v.status = itst + B2;
+  z** pa[13];
+  for (int i = 0; i < 13; ++i) pa[i] = (z**)take(B2 * sizeof(z*));
+  hipStream_t st = v.stream;
+  const unsigned g = (unsigned)((B2 + 255) / 256);
+  auto fill = [&](z** out, z* b, long long stride, int count) {
+    fill_ptrs_kernel<<<g, 256, 0, st>>>(out, b, stride, count, B2);
+  };
+  fill(pa[0], (z*)ml, (long long)nn, nb);
+  fill(pa[1], (z*)mr, (long long)nn, nb);
+  fill(pa[2], (z*)m0, (long long)nn, nb);
+  fill(pa[3], eps, (long long)nn, B2);
+  fill(pa[4], xs, (long long)nn, B2);
+  fill(pa[5], Yinv, (long long)nn, B2);
+  fill(pa[6], AYBY, 2 * (long long)nn, B2);
+  for (int h = 0; h < 2; ++h) {
+    z* abh = AB + (size_t)h * B2 * 2 * nn;
+    fill(pa[7 + h], abh, 2 * (long long)nn, B2);       // [alpha; beta] (and alpha)
+    fill(pa[9 + h], abh + n, 2 * (long long)nn, B2);   // beta
+  }
+  fill_int_kernel<<<g, 256, 0, st>>>(v.active, 1, nb, B2);
+  fill_int_kernel<<<g, 256, 0, st>>>(v.iters, 0, 0, 2 * B2);  // iters and status zeroed
+  HIPCHK(hipGetLastError());
+  v.p_ml.d = pa[0];
+  v.p_mr.d = pa[1];
+  v.p_m0.d = pa[2];
+  v.p_eps.d = pa[3];
+  v.p_x.d = pa[4];
+  v.p_Yinv.d = pa[5];
+  v.p_AYBY.d = pa[6];
+  v.p_AB[0].d = v.p_alpha[0].d = pa[7];
+  v.p_AB[1].d = v.p_alpha[1].d = pa[8];
+  v.p_beta[0].d = pa[9];
+  v.p_beta[1].d = pa[10];
+  int r = run_decimation(&v, false, false);
+  // growth flagged in a blocked inversion: the chains again with full pivoting
+  int hs[1024];
+  if (r == SKKR_OK) {
+    HIPCHK(hipMemcpyAsync(hs, v.status, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool redo = false;
+    for (int j = 0; j < nb; ++j) redo = redo || (hs[j] & GJ_STATUS_REDO);
+    if (redo) {
+      fill_int_kernel<<<g, 256, 0, st>>>(v.active, 1, nb, B2);
+      fill_int_kernel<<<g, 256, 0, st>>>(v.iters, 0, 0, 2 * B2);
+      v.inv_kernel = 1;
+      r = run_decimation(&v, false, false);
+    }
+  }
+  if (r == SKKR_OK) {
+    HIPCHK(hipMemcpyAsync(x, Yinv, sizeof(z) * nn * nb, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(iters, v.iters, sizeof(int) * nb, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(hs, v.status, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j < nb; ++j) {
+      hs[j] = status_code(hs[j]);
+      if (hs[j] != SKKR_OK && r == SKKR_OK) r = hs[j];
+    }
+    HIPCHK(hipMemcpyAsync(status, hs, sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  v.stream = nullptr;  // the caller's stream: not destroyed with the view
+  return r;
 }
 
 int skkr_dugo_batch(skkr_ctx* c, char side, int nb, const double* m00, const double* m01,

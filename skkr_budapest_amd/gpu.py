@@ -25,7 +25,8 @@ LIB_PATH = os.path.join(_HERE, "libskkr_gpu.so")
 SKKR_OK = 0
 ERRORS = {1: "SKKR_ERR_ARG", 2: "SKKR_ERR_HIP", 3: "SKKR_ERR_NOMEM",
           4: "SKKR_ERR_SINGULAR", 5: "SKKR_ERR_SGF_DIVERGED",
-          6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED", 8: "SKKR_ERR_STR_NOCONV"}
+          6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED", 8: "SKKR_ERR_STR_NOCONV",
+          9: "SKKR_ERR_TOO_LARGE"}
 FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
 
 # exported symbols (every declaration of include/skkr_gpu.h)
@@ -35,7 +36,8 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_tau2d_batch", "skkr_set_option", "skkr_autotune", "skkr_set_screening",
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
            "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs",
-           "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch", "skkr_set_point_group")
+           "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch", "skkr_set_point_group",
+           "skkr_surfgfd_noalloc")
 
 
 # codes that name a failing k-point or matrix (reported per item in status
@@ -126,6 +128,7 @@ def lib():
                                   ctypes.POINTER(Stats)]
         L.skkr_set_pairs.argtypes = [V, I, V, V, V]
         L.skkr_tau_bz_pairs.argtypes = [V, D, D, I, V, V, V, V, V, V, V, ctypes.POINTER(Stats)]
+        L.skkr_surfgfd_noalloc.argtypes = [I, I, I, V, V, V, V, V, V, D, I, V]
         _LIB = L
     return _LIB
 
