@@ -6,7 +6,10 @@ both leads, Godfrin sweeps, BZ accumulation), inputs resident in HBM.
 
 Default workload: the relativistic layer system with lk=32, 4 sites per
 principal layer (n=128), 20 interface PLs, 25 in-plane vectors, one energy
-E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points).
+E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points), timed first in FP64 and
+then in the split-BF16 mixed mode; `value` is the mixed mode's k-points/s,
+`config` holds both rates, their ratio and the mixed result's error against
+the FP64 one (tau_BZ, tr G, DOS).  --fp64-only times FP64 alone.
 
 --kkr runs the k-loop on the reference's own structure constants instead of
 the synthetic Fourier sum: the Nb(110) stack of sample/02-layer-scf (lmax 2,
@@ -46,6 +49,13 @@ def algorithmic_flops_per_k(n, P, it_l, it_r):
     return ((56 * P - 48) + (56 * it_l + 24) + (56 * it_r + 24)) * float(n) ** 3
 
 
+# k-points/s of the reference's own compiled k-loop (flang -O2, 8 processes,
+# round-robin k) over those of the C oracle (8 threads) on the same cores, for
+# the n = 64 stack, the largest the reference's param.h admits
+# (profiles/cpu_reference_vs_port_cfg0_n64.md, oracle/ref/time_cfg0.py)
+REFERENCE_OVER_PORT = 1.20
+
+
 def cpu_baseline(sy, E, k, w):
     import oracle
     nworkers = min(16, os.cpu_count() or 1)
@@ -56,7 +66,9 @@ def cpu_baseline(sy, E, k, w):
     dt = time.perf_counter() - t0
     return {"value": nk / dt, "unit": "k-points/s", "cores": nworkers, "kind": "port",
             "sample": f"{nk} k-points of the same energy and mesh, FP64 C oracle, "
-                      f"{nworkers} threads, round-robin k split, {dt:.1f} s"}
+                      f"{nworkers} threads, round-robin k split, {dt:.1f} s",
+            "reference_over_port": REFERENCE_OVER_PORT,
+            "reference_equivalent_value": REFERENCE_OVER_PORT * nk / dt}
 
 
 def contour_bench(args, cfg, sy):
