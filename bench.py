@@ -332,6 +332,9 @@ def main():
     runs = {}
     for name, prec in modes:
         ctx = gpu.Context.for_system(sy, precision=prec)
+        # every launch bracketed by events (the sweep graphs hide theirs; on
+        # this workload replays and launches take the same time)
+        ctx.set_option("graphs", 0)
         ctx.set_tinv(*sy.tinv(E))
         runs[name] = kloop_steps(ctx, E, k, w, args, name)
         ctx.close()
