@@ -9,8 +9,10 @@
 // pivoting restricted to the rows of the diagonal block (rows are never
 // exchanged between blocks).  Per block column K with diagonal block D:
 //   1. block column K -> LDS;
-//   2. wave 0 inverts D in LDS (in-place Gauss-Jordan, partial pivoting within
-//      the 16 rows, then unscramble) -> Dinv;
+//   2. the wave holding D inverts it in registers (Gauss-Jordan with partial
+//      pivoting within the 16 rows, pivot search and broadcasts by DPP and
+//      permlane swaps, then unscramble) -> Dinv; from the second block
+//      column on this overlaps the other waves' rank update of step K-1;
 //   3. W = -M[O,K] Dinv for the rows O outside block K, W[K] = Dinv, and
 //      X = Dinv R for R the rows of block K in the other block columns, as
 //      16x16 MFMA tile products, one tile per wave;
@@ -33,21 +35,17 @@
 constexpr int GJ_STATUS_REDO = 16;
 constexpr double GJ_GROWTH_LIMIT = 1048576.0;  // 2^20
 
-// 16x16 Gauss-Jordan inversion of a diagonal block held by one wave in the
-// v_mfma_f64_16x16x4 result layout (lane l, element q: row l/16 + 4q, column
-// l%16), with partial pivoting within the 16 rows.  The pivot search, the
-// pivot row and the pivot column move between lanes by shuffles.  Writes the
-// unscrambled inverse to dinv[col * DP + row]; sets *sing on a zero or NaN
-// pivot.
+// Shuffle of a complex value: lane `src` of the wave (ds_bpermute)
 __device__ __forceinline__ double2 shfl2(double2 v, int src) {
   return make_double2(__shfl(v.x, src), __shfl(v.y, src));
 }
-// value of lane `src` (wave-uniform) in every lane, without an LDS round trip
-__device__ __forceinline__ double readlane_f64(double v, int src) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+
+// Lane IT of each row of 16 lanes, in every lane of that row (DPP
+// row_newbcast: a VALU move, no LDS round trip)
+template <int IT>
+__device__ __forceinline__ double2 row_bcast(double2 v) {
+  return make_double2(__builtin_amdgcn_update_dpp(0.0, v.x, 0x150 + IT, 0xF, 0xF, true),
+                      __builtin_amdgcn_update_dpp(0.0, v.y, 0x150 + IT, 0xF, 0xF, true));
 }
 
 // 1/z for a pivot z: conj(z) / |z|^2 with the reciprocal of |z|^2 from
@@ -61,26 +59,105 @@ __device__ __forceinline__ double2 pivot_recip(double2 z) {
   return make_double2(z.x * r, -z.y * r);
 }
 
-// One elimination step on element q of a lane: row l4 + 4q of column l16.
+// Pivot key of a candidate z in row `row`: the high word of |z|^2 (sign,
+// exponent, 20 mantissa bits) with its four lowest bits replaced by
+// 15 - row, compared as a signed integer -- the larger magnitude wins, the
+// smaller row on a tie to 2^-16 relative; -1 for a row already pivoted.
+// Zero magnitudes give keys 0..15, infinities and NaN keys >= 0x7ff00000.
+__device__ __forceinline__ int pivot_key(double2 z, int row, bool used) {
+  const double m = z.x * z.x + z.y * z.y;
+  const int k = (__double2hiint(m) & ~15) | (15 - row);
+  return used ? -1 : k;
+}
+
+// The larger of this lane's key and that of lane l ^ 32 (l ^ 16): the swap
+// returns the two halves exchanged between two copies of the key, one of
+// which is this lane's own value and the other its partner's.
+__device__ __forceinline__ int max_xor32(int k) {
+  const auto r = __builtin_amdgcn_permlane32_swap(k, k, false, false);
+  return max((int)r[0], (int)r[1]);
+}
+__device__ __forceinline__ int max_xor16(int k) {
+  const auto r = __builtin_amdgcn_permlane16_swap(k, k, false, false);
+  return max((int)r[0], (int)r[1]);
+}
+
+// One elimination step on element q of a lane (row r of column l16), as
+// selects rather than branches: the pivot row p takes rn = prow / piv, the
+// pivot column starts from zero, every other element subtracts f_r rn.
 __device__ __forceinline__ void gj_update(double2& dq, int r, int p, bool pivcol, double2 fq,
                                           double2 rn) {
-  if (r == p) {
-    dq = rn;
-  } else {
-    double2 o = pivcol ? make_double2(0.0, 0.0) : dq;
-    o.x -= fq.x * rn.x - fq.y * rn.y;
-    o.y -= fq.x * rn.y + fq.y * rn.x;
-    dq = o;
+  double2 o = pivcol ? make_double2(0.0, 0.0) : dq;
+  o.x = fma(-fq.x, rn.x, fma(fq.y, rn.y, o.x));
+  o.y = fma(-fq.x, rn.y, fma(-fq.y, rn.x, o.y));
+  dq = (r == p) ? rn : o;
+}
+
+// Step IT of the 16x16 inversion below.  The column IT of the lane's four
+// rows comes from lane IT of its row of lanes (DPP); the pivot is the
+// largest key over the four rows of the lane and the four rows of lanes
+// (permlane swaps), read into a scalar register; the pivot row p moves to
+// every row of lanes by one shuffle and its column-IT element, the pivot,
+// by DPP again.  sperm[IT] = p and spinv[p] = IT go to LDS (every lane
+// stores the same value).
+template <int IT, bool PIV>
+__device__ __forceinline__ void gj16_step(double2& d0, double2& d1, double2& d2, double2& d3, int l16,
+                                          int l4, unsigned& used, bool& bad, int* sperm, int* spinv) {
+  const double2 f0 = row_bcast<IT>(d0), f1 = row_bcast<IT>(d1);
+  const double2 f2 = row_bcast<IT>(d2), f3 = row_bcast<IT>(d3);
+  int p = IT;
+  if constexpr (PIV) {
+    const int k0 = pivot_key(f0, l4, (used >> l4) & 1u);
+    const int k1 = pivot_key(f1, l4 + 4, (used >> (l4 + 4)) & 1u);
+    const int k2 = pivot_key(f2, l4 + 8, (used >> (l4 + 8)) & 1u);
+    const int k3 = pivot_key(f3, l4 + 12, (used >> (l4 + 12)) & 1u);
+    const int kb =
+        __builtin_amdgcn_readfirstlane(max_xor16(max_xor32(max(max(k0, k1), max(k2, k3)))));
+    if (kb < 0) {  // no unused row (not reachable before step 16)
+      p = __builtin_ctz(~used & 0xFFFFu);
+      bad = true;
+    } else {
+      p = 15 - (kb & 15);
+      if (kb < 16 || kb >= 0x7ff00000) bad = true;  // zero, infinite or NaN pivot
+    }
+    used |= 1u << p;
+    sperm[IT] = p;
+    spinv[p] = IT;
+  }
+  // pivot row p of this lane's column: lane (p&3)*16 + l16, element p>>2
+  const int qp = p >> 2;
+  const double2 mine = qp == 0 ? d0 : qp == 1 ? d1 : qp == 2 ? d2 : d3;
+  const double2 prow = shfl2(mine, (p & 3) * 16 + l16);
+  const double2 piv = row_bcast<IT>(prow);
+  if constexpr (!PIV) {
+    if (!(piv.x * piv.x + piv.y * piv.y > 0.0)) bad = true;  // zero or NaN pivot
+  }
+  const double2 qv = pivot_recip(piv);
+  const bool pivcol = (l16 == IT);
+  const double2 rn = pivcol ? qv : cmul(prow, qv);
+  gj_update(d0, l4, p, pivcol, f0, rn);
+  gj_update(d1, l4 + 4, p, pivcol, f1, rn);
+  gj_update(d2, l4 + 8, p, pivcol, f2, rn);
+  gj_update(d3, l4 + 12, p, pivcol, f3, rn);
+}
+
+template <bool PIV, int IT = 0>
+__device__ __forceinline__ void gj16_steps(double2& d0, double2& d1, double2& d2, double2& d3, int l16,
+                                           int l4, unsigned& used, bool& bad, int* sperm, int* spinv) {
+  if constexpr (IT < 16) {
+    gj16_step<IT, PIV>(d0, d1, d2, d3, l16, l4, used, bad, sperm, spinv);
+    gj16_steps<PIV, IT + 1>(d0, d1, d2, d3, l16, l4, used, bad, sperm, spinv);
   }
 }
 
 // 16x16 Gauss-Jordan inversion of a diagonal block held by one wave in the
 // v_mfma_f64_16x16x4 result layout (lane l, element q: row l/16 + 4q, column
-// l%16), with partial pivoting within the 16 rows.  The elements are four
-// separate variables (an index into an array by the wave-uniform pivot row
-// would be lowered to scratch memory).  Pivot search by lane reads, pivot
-// row and column by shuffles.  Writes the unscrambled inverse to
-// dinv[col * DP + row]; sets *sing on a zero or NaN pivot.
+// l%16), with partial pivoting within the 16 rows (pivot rows recorded, no
+// exchanges).  The elements are four separate variables (an index into an
+// array by the wave-uniform pivot row would be lowered to scratch memory);
+// the sixteen steps are unrolled so that the DPP lane selects are
+// immediates.  Writes the unscrambled inverse to dinv[col * DP + row]; sets
+// *sing on a zero, infinite or NaN pivot.
 template <int DP, bool PIV>
 __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2& d2, double2& d3,
                                                int lane, double2* __restrict__ dinv, int* sperm,
@@ -88,63 +165,7 @@ __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2
   const int l16 = lane & 15, l4 = lane >> 4;
   unsigned used = 0;  // rows already pivoted (identical in every lane)
   bool bad = false;
-#pragma unroll 1
-  for (int it = 0; it < 16; ++it) {
-    // column it of this lane's rows (independent of the pivot: issued first)
-    const int csrc = l4 * 16 + it;
-    const double2 f0 = shfl2(d0, csrc), f1 = shfl2(d1, csrc);
-    const double2 f2 = shfl2(d2, csrc), f3 = shfl2(d3, csrc);
-    int p = it;
-    if constexpr (PIV) {
-    // candidates of column it: rows l4 + 4q of lane l4*16 + it
-    double best = -1.0;
-    p = 16;
-    {
-      const double m0 = d0.x * d0.x + d0.y * d0.y, m1 = d1.x * d1.x + d1.y * d1.y;
-      const double m2 = d2.x * d2.x + d2.y * d2.y, m3 = d3.x * d3.x + d3.y * d3.y;
-      if (!((used >> l4) & 1u) && m0 > best) { best = m0; p = l4; }
-      if (!((used >> (l4 + 4)) & 1u) && m1 > best) { best = m1; p = l4 + 4; }
-      if (!((used >> (l4 + 8)) & 1u) && m2 > best) { best = m2; p = l4 + 8; }
-      if (!((used >> (l4 + 12)) & 1u) && m3 > best) { best = m3; p = l4 + 12; }
-    }
-    // largest over the four lanes of column it (read lane by lane, so the
-    // choice is wave-uniform); the first row wins ties
-    double bb = readlane_f64(best, it);
-    int pb = __builtin_amdgcn_readlane(p, it);
-#pragma unroll
-    for (int m = 1; m < 4; ++m) {
-      const double ob = readlane_f64(best, it + 16 * m);
-      const int op = __builtin_amdgcn_readlane(p, it + 16 * m);
-      if (ob > bb || (ob == bb && op < pb)) { bb = ob; pb = op; }
-    }
-    p = pb;
-    if (p > 15) {  // only with NaN entries: take the first unused row
-      p = __builtin_ctz(~used & 0xFFFFu);
-      bad = true;
-    }
-    if (!(bb > 0.0)) bad = true;
-    used |= 1u << p;
-    if (lane == 0) {
-      sperm[it] = p;
-      spinv[p] = it;
-    }
-    }
-    // pivot row p of this lane's column: lane (p&3)*16 + l16, element p>>2
-    const int qp = p >> 2;
-    const double2 mine = qp == 0 ? d0 : qp == 1 ? d1 : qp == 2 ? d2 : d3;
-    const double2 prow = shfl2(mine, (p & 3) * 16 + l16);
-    const double2 piv = make_double2(readlane_f64(prow.x, it), readlane_f64(prow.y, it));
-    if constexpr (!PIV) {
-      if (!(piv.x * piv.x + piv.y * piv.y > 0.0)) bad = true;  // zero or NaN pivot
-    }
-    const double2 qv = pivot_recip(piv);
-    const bool pivcol = (l16 == it);
-    const double2 rn = pivcol ? qv : cmul(prow, qv);
-    gj_update(d0, l4, p, pivcol, f0, rn);
-    gj_update(d1, l4 + 4, p, pivcol, f1, rn);
-    gj_update(d2, l4 + 8, p, pivcol, f2, rn);
-    gj_update(d3, l4 + 12, p, pivcol, f3, rn);
-  }
+  gj16_steps<PIV>(d0, d1, d2, d3, l16, l4, used, bad, sperm, spinv);
   if (bad && lane == 0) *sing = 1;
   if constexpr (PIV) {
     // unscramble: Dinv[pinv(rho)][perm(gam)] = S[rho][gam]
@@ -164,7 +185,7 @@ __device__ __forceinline__ void diag_inverse16(double2& d0, double2& d1, double2
 }
 
 template <int T, int TI, int TJ, bool PIV = true>
-__global__ __launch_bounds__((T / TI) * (T / TJ) * 64) void gjinv_blk_f64_kernel(InvArgs args) {
+__global__ __launch_bounds__((T / TI) * (T / TJ) * 64, 2) void gjinv_blk_f64_kernel(InvArgs args) {
   constexpr int N = 16 * T;
   constexpr int WC = T / TJ;
   constexpr int NW = (T / TI) * WC;
