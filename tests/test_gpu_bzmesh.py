@@ -82,3 +82,44 @@ def test_learned_mesh_on_held_out_layer_systems():
         learned += r["learned"][0]
         probe += r["probe"][0]
     assert 2 * learned < probe
+
+
+def test_learned_mesh_on_the_config4_system():
+    """The config-4 system itself (lk = 32, 4 sites per PL, n = 128, 20 PLs,
+    E = 0.6 + 0.005i) through the GPU k-loop, at 2e-5 relative error from the
+    32 x 32 triangulation with up to 4 levels (512 x 512).  The judge is
+    uniform level 5 (1024 x 1024), a level finer than anything judged and
+    evaluated on its own; its error (~1e-7 against level 6 in the full
+    benchmark) is far below the tolerance.  Uniform refinement first reaches
+    2e-5 at level 4 (262,144 points: level 3 is at 2.5e-5); the learned
+    proposal, with the indicator threshold raised tenfold for the looser
+    tolerance, needs about half (130,944 points measured, error 1.8e-5).
+    At coarser settings (1e-4, up to level 3) the proposal refines nearly
+    every cell and gains nothing: the gain of the learned mesh is in the deep
+    levels."""
+    from skkr_budapest_amd import bzmesh as BM
+    from skkr_budapest_amd import bztrain as BT
+    from skkr_budapest_amd import gpu
+    from skkr_budapest_amd import systems as S
+    cfg = S.CONFIGS[4]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    tol, n0, lmax = 2e-5, 32, 4
+    print(f"bzmesh: config 4 system n={cfg.n} P={cfg.P} E={E}", flush=True)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ev = BT.trace_evaluator(ctx, E)
+        lr = BM.WindowedMesh(ev, n0, lmax, E.imag).integrate(
+            "learned", theta=10 * BT.THETA, predict=_model().predict, margin=BT.MARGIN)
+        print(f"bzmesh: learned {lr['n_evals']} points", flush=True)
+        judge = BM.uniform_levels(ev, n0, lmax + 1)
+        print(f"bzmesh: judge {judge[-1][1]} points", flush=True)
+    ref = judge[-1][0]
+    rel = lambda x: abs(x - ref) / abs(ref)
+    errs = [rel(I) for I, _ in judge[:-1]]
+    print(f"bzmesh: uniform errors {['%.2e' % e for e in errs]}, learned err {rel(lr['integral']):.2e}",
+          flush=True)
+    assert errs[3] > tol >= errs[4]        # uniform needs level 4
+    need = judge[4][1]
+    assert rel(lr["integral"]) <= tol
+    assert lr["n_evals"] <= 0.6 * need
