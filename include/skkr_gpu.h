@@ -246,6 +246,51 @@ int skkr_tau2d_layered_batch(skkr_ctx *ctx, int nb, const double *m00,
                              const double *x, const double *y, double w,
                              double *tau, int *status);
 
+/* tau2d with the reference's own arguments, for one k-point
+ * (skkr_kpar_v6/main/tau2d.f:2-3):
+ *   tau2d(irel, wgeff, bulkgeo, lmax, lkmax, lkmaxp, nintfc, tminv, g2d, x, y, tau, mdimp)
+ * followed by what tau2d takes from param.h and its common blocks: the
+ * dimensions lmmaxp, minprc, mprc1, mintfc of g2d(lmmaxp, lmmaxp,
+ * -minprc:minprc, minprc, 0:mprc1+1), tminv and tau(lkmaxp, lkmaxp, mintfc);
+ * nprc and ninprc(0:nprc+1) of common /lay2d/; fac of common /relfac/ (read
+ * when irel = 1).  Arrays are the reference's complex*16 arrays as declared
+ * (column-major, padded); x and y are (mdimp, mdimp) with the leading
+ * lkmax*ninprc(1) square used.  The packer (packer.f:6-100, relmtrx.f for
+ * irel = 1) runs on the GPU from the g2d planes (plane 1 for every PL with
+ * bulkgeo, plane p for PL p otherwise), then the sweeps of skkr_tau2d_batch
+ * or skkr_tau2d_layered_batch.  tau(:, :, 1:nintfc) is written, the rest of
+ * tau is left as it is.  The context must have lk = lkmax, n_pl = nprc and
+ * sites_per_pl = ninprc(p) for p = 1..nprc (SKKR_ERR_UNSUPPORTED for
+ * unequal principal layers); tminv replaces its interface t^-1, as
+ * skkr_set_tinv would.  Returns SKKR_ERR_SINGULAR as gjinv would stop. */
+int skkr_tau2d_g2d(skkr_ctx *ctx, int irel, double wgeff, int bulkgeo,
+                   int lmax, int lkmax, int lkmaxp, int nintfc,
+                   const double *tminv, const double *g2d, const double *x,
+                   const double *y, double *tau, int mdimp, int lmmaxp,
+                   int minprc, int mprc1, int mintfc, int nprc,
+                   const int *ninprc, double fac_re, double fac_im);
+
+/* dugo with the reference's own arguments, for one k-point
+ * (skkr_kpar_v6/main/dugo.f:9-10):
+ *   dugo(lmax, lkmax, tminvb, lkmaxp, g2d, g2di, xy, mdimp, side, bulk, irel, sgfconv)
+ * followed by lmmaxp, minprc (param.h), nprc and ninprc(0:nprc+1) (common
+ * /lay2d/), fac (common /relfac/) and iters (out, optional: the decimation
+ * iterations).  g2d and g2di are (lmmaxp, lmmaxp, -minprc:minprc, minprc)
+ * windows; tminvb is (lkmaxp, lkmaxp, minprc).  Both branches of dugo.f:44-72:
+ * bulk (the lead's blocks from g2di, the outer factor from g2d) and not bulk
+ * (the lead's blocks from g2d, the interface PL's coupling from g2di).  The
+ * decimation stops on the context's errmax and itermax_l / itermax_r (the
+ * values of common /iterpar/).  xy(1:n, 1:n), n = lkmax*ninprc(1 or nprc),
+ * is written; sgfconv = 0 when the decimation diverged (surfgf.f:89-94),
+ * which is also the return code; SKKR_ERR_SGF_ITERMAX where surfgf.f stops.
+ * tminvb replaces the context's lead t^-1 of that side. */
+int skkr_dugo_g2d(skkr_ctx *ctx, int lmax, int lkmax, const double *tminvb,
+                  int lkmaxp, const double *g2d, const double *g2di,
+                  double *xy, int mdimp, char side, int bulk, int irel,
+                  int *sgfconv, int lmmaxp, int minprc, int nprc,
+                  const int *ninprc, double fac_re, double fac_im,
+                  int *iters);
+
 /* Point group of an irreducible-wedge k-mesh (intbz = 1,
  * skkr_kpar_v6/nrsp-scf/cpacoord.f:98-101,281-299): the BZ sums of
  * skkr_tau_bz, skkr_tau_bz_phys and skkr_cpa_bz become

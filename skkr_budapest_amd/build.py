@@ -66,13 +66,16 @@ def build_fortran(verbose=False):
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
     if verbose:
         print("built", exe)
-    # the structure-constant producer and tau2d called from Fortran
-    for src, name in (("skkr_gstore_main.f90", "skkr_gstore"), ("skkr_tau2d_main.f90", "skkr_tau2d")):
+    # the structure-constant producer, tau2d, and the reference-shaped dugo /
+    # tau2d (skkr_dropin.f90) called from Fortran
+    for extra, src, name in (((), "skkr_gstore_main.f90", "skkr_gstore"),
+                             ((), "skkr_tau2d_main.f90", "skkr_tau2d"),
+                             (("skkr_dropin.f90",), "skkr_g2d_main.f90", "skkr_g2d")):
         gsrc = os.path.join(FSRC, src)
         if not os.path.exists(gsrc):
             continue
         gexe = os.path.join(FSRC, name)
-        cmd = [fc, "-O2", "-J", bdir, srcs[0], gsrc, "-o", gexe,
+        cmd = [fc, "-O2", "-J", bdir, srcs[0], *(os.path.join(FSRC, e) for e in extra), gsrc, "-o", gexe,
                "-L" + HERE, "-lskkr_gpu", "-Wl,-rpath," + HERE]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -158,6 +158,12 @@ struct skkr_ctx {
   int str_rel = 0;                // 0 lm basis; 1, 2 kappa-mu (psq = E + E^2/c^2, or E)
   double *dU1 = nullptr, *dU2 = nullptr;
   int *dI1 = nullptr, *dI2 = nullptr;
+  // g2d windows of the reference-shaped tau2d / dugo entries
+  z* dG2d = nullptr;
+  size_t g2d_cap = 0;
+  z* dMt = nullptr;  // dugo's extra coupling block (one chain)
+  PtrArr p_dmt;      // every chain -> dMt
+  int dmt_cap = 0;   // chains of p_dmt
   int mt_cap = 0;
   double e_re = 0.0, e_im = 0.0;  // energy of the current tau_bz call
   // CPA loop: 1 = keep the batch's blocks and lead self-energies, 2 = reuse
@@ -659,7 +665,8 @@ static int run_fourier(skkr_ctx* c, int nk) {
 // updates of eps, x and the next [alpha; beta] fused into the GEMM epilogues.
 // Leaves g_s = x^-1 in Yinv and the self-energy ml g_s mr in p_sigma when
 // want_sigma (ml g_s mt with the outer factors mt when given).
-static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const PtrArr* mt = nullptr) {
+static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const PtrArr* mt = nullptr,
+                          const PtrArr* lt = nullptr) {
   const int n = c->n, B2 = 2 * c->B;
   dec_init_kernel<<<B2, 256, 0, c->stream>>>(n, c->cfg.lk, c->p_ml.d, c->p_mr.d, c->p_m0.d,
                                              add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
@@ -744,9 +751,9 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
   ia.status = c->status;
   CHK(run_inverse(c, ia, B2));
   if (want_sigma) {
-    // T = ml g_s (into AYBY as n x n), sigma = T mr
-    CHK(launch_zgemm(c, n, n, n, c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero, B2,
-                     c->active));
+    // T = ml g_s (into AYBY as n x n), sigma = T mr (lt / mt in place of ml / mr)
+    CHK(launch_zgemm(c, n, n, n, lt ? lt->d : c->p_ml.d, n, c->p_Yinv.d, n, c->p_AYBY.d, n, one, zero,
+                     B2, c->active));
     CHK(launch_zgemm(c, n, n, n, c->p_AYBY.d, n, mt ? mt->d : c->p_mr.d, n, c->p_sigma.d, n, one, zero, B2,
                      c->active));
   }
@@ -1639,6 +1646,58 @@ int skkr_zgemm_batch(skkr_ctx* c, int n, int nb, const double* a, const double* 
 }
 
 // decimation of nb chains given by host matrices; lead t^-1 added for dugo
+// Decimation of nb chains whose blocks are already in the device buffers
+// (left lead: M10/M00/M01 entries 0..nb-1; right lead: M01/M00/M10, chains
+// B..B+nb-1), again with the fully pivoting inversion when a blocked one
+// flagged growth.  Leaves g_s in Yinv and, with want_sigma, the self-energy
+// in Ycur (left) or Xc (right).
+static int dec_device(skkr_ctx* c, int nb, bool add_tinv, char side, bool want_sigma,
+                      const PtrArr* mt = nullptr, const PtrArr* lt = nullptr) {
+  const int B = c->B;
+  const int off = (side == 'R') ? B : 0;
+  std::vector<int> act(2 * B, 0);
+  for (int j = 0; j < nb; ++j) act[off + j] = 1;
+  HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+  HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+  CHK(run_decimation(c, add_tinv, want_sigma, mt, lt));
+  // growth flagged in a blocked inversion: the chains again, pivoting over all rows
+  std::vector<int> hs0(2 * B);
+  HIPCHK(hipMemcpyAsync(hs0.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  bool redo = false;
+  for (int j = 0; j < nb; ++j) redo = redo || (hs0[off + j] & GJ_STATUS_REDO);
+  if (redo) {
+    const int saved = c->inv_kernel;
+    c->inv_kernel = 1;
+    HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
+    HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
+    const int r = run_decimation(c, add_tinv, want_sigma, mt, lt);
+    c->inv_kernel = saved;
+    CHK(r);
+  }
+  return SKKR_OK;
+}
+
+// iteration counts and status codes of the nb chains of one side
+static int dec_results(skkr_ctx* c, int nb, char side, int* iters, int* status) {
+  const int B = c->B;
+  const int off = (side == 'R') ? B : 0;
+  std::vector<int> hi(2 * B), hs(2 * B);
+  HIPCHK(hipMemcpyAsync(hi.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(hs.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = SKKR_OK;
+  for (int j = 0; j < nb; ++j) {
+    int code = status_code(hs[off + j]);
+    if (iters) iters[j] = hi[off + j];
+    if (status) status[j] = code;
+    if (code != SKKR_OK && rc == SKKR_OK) rc = code;
+  }
+  return rc;
+}
+
 static int dec_host(skkr_ctx* c, int nb, const double* ml, const double* m0, const double* mr,
                     bool add_tinv, char side, double* x_out, double* sigma_out, int* iters,
                     int* status) {
@@ -1653,48 +1712,14 @@ static int dec_host(skkr_ctx* c, int nb, const double* ml, const double* m0, con
   HIPCHK(hipMemcpyAsync(mlbuf, ml, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(mrbuf, mr, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->M00, m0, sizeof(z) * nn * nb, hipMemcpyHostToDevice, c->stream));
-  std::vector<int> act(2 * B, 0);
-  for (int j = 0; j < nb; ++j) act[off + j] = 1;
-  HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
-  HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
-  CHK(run_decimation(c, add_tinv, sigma_out != nullptr));
-  {
-    // growth flagged in a blocked inversion: the chains again, pivoting over all rows
-    std::vector<int> hs0(2 * B);
-    HIPCHK(hipMemcpyAsync(hs0.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    bool redo = false;
-    for (int j = 0; j < nb; ++j) redo = redo || (hs0[off + j] & GJ_STATUS_REDO);
-    if (redo) {
-      const int saved = c->inv_kernel;
-      c->inv_kernel = 1;
-      HIPCHK(hipMemcpyAsync(c->active, act.data(), 2 * B * sizeof(int), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemsetAsync(c->iters, 0, 2 * B * sizeof(int), c->stream));
-      HIPCHK(hipMemsetAsync(c->status, 0, 2 * B * sizeof(int), c->stream));
-      const int r = run_decimation(c, add_tinv, sigma_out != nullptr);
-      c->inv_kernel = saved;
-      CHK(r);
-    }
-  }
+  CHK(dec_device(c, nb, add_tinv, side, sigma_out != nullptr));
   if (x_out)
     HIPCHK(hipMemcpyAsync(x_out, c->Yinv + off * nn, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   if (sigma_out) {
     z* src = (side == 'R') ? c->Xc : c->Ycur;
     HIPCHK(hipMemcpyAsync(sigma_out, src, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   }
-  std::vector<int> hi(2 * B), hs(2 * B);
-  HIPCHK(hipMemcpyAsync(hi.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(hs.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  int rc = SKKR_OK;
-  for (int j = 0; j < nb; ++j) {
-    int code = status_code(hs[off + j]);
-    if (iters) iters[j] = hi[off + j];
-    if (status) status[j] = code;
-    if (code != SKKR_OK && rc == SKKR_OK) rc = code;
-  }
-  return rc;
+  return dec_results(c, nb, side, iters, status);
 }
 
 int skkr_surfgfd_batch(skkr_ctx* c, int nb, const double* ml, const double* m0,
@@ -1979,6 +2004,254 @@ int skkr_tau2d_layered_batch(skkr_ctx* c, int nb, const double* m00, const doubl
     if (status) status[j] = code;
     if (code != SKKR_OK && rc == SKKR_OK) rc = code;
   }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// tau2d and dugo with the reference's own arrays (g2d windows, padded leading
+// dimensions), one k-point per call as the reference calls them
+// ---------------------------------------------------------------------------
+
+// Shape of the reference's g2d(lmmaxp, lmmaxp, -minprc:minprc, minprc) window
+struct G2dShape {
+  int lmmaxp, minprc;
+  size_t window() const { return (size_t)lmmaxp * lmmaxp * (2 * minprc + 1) * minprc; }
+};
+
+// packer.f:6-100 for a principal layer of s layers between neighbours of s
+// layers (ni = nim = nip = s): element (r, c) of the n x n blocks, n = lk s,
+//   M00 = -G(i, i+j),   j = jp - i          (every layer pair of the PL),
+//   M10 = -G(i, j),     j = jp - i - s     (jp >= i: the PL before),
+//   M01 = -G(i, j),     j = jp - i + s     (jp <= i: the PL after),
+// with G(i, j) the lm block g2d(:, :, j, i) (i 1-based).  irel = 1: every
+// block through relmtrx.f:26-42, b(r, c) = u1(r) (f a)(ind1 r, ind1 c) u1(c)
+// + u2(r) (f a)(ind2 r, ind2 c) u2(c), indices past the lm block reading 0.
+// Null outputs are skipped.
+__global__ __launch_bounds__(256) void g2d_pack_kernel(int s, int lk, int nl2, G2dShape sh,
+                                                       const double2* __restrict__ win, int rel, double fr,
+                                                       double fi, const double* __restrict__ u1,
+                                                       const double* __restrict__ u2, const int* __restrict__ i1,
+                                                       const int* __restrict__ i2,
+                                                       double2* M00, double2* M10, double2* M01) {
+  const int n = lk * s;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long long)n * n) return;
+  const int r = (int)(idx % n), c = (int)(idx / n);
+  const int i0 = r / lk, kr = r % lk, jp0 = c / lk, kc = c % lk;
+  const int ld = sh.lmmaxp, nj = 2 * sh.minprc + 1;
+  auto elem = [&](int a, int b, int j) {  // g2d(a, b, j, i0 + 1), a, b 0-based
+    return win[a + (size_t)ld * (b + (size_t)ld * ((j + sh.minprc) + (size_t)nj * i0))];
+  };
+  auto blk = [&](int j) {
+    if (!rel) {
+      const double2 v = elem(kr, kc, j);
+      return make_double2(-v.x, -v.y);
+    }
+    double2 acc = make_double2(0.0, 0.0);
+    const int p1 = i1[kr], q1 = i1[kc], p2 = i2[kr], q2 = i2[kc];
+    if (p1 < nl2 && q1 < nl2) {
+      const double2 v = elem(p1, q1, j);
+      const double cf = u1[kr] * u1[kc];
+      acc.x += cf * (fr * v.x - fi * v.y);
+      acc.y += cf * (fr * v.y + fi * v.x);
+    }
+    if (p2 < nl2 && q2 < nl2) {
+      const double2 v = elem(p2, q2, j);
+      const double cf = u2[kr] * u2[kc];
+      acc.x += cf * (fr * v.x - fi * v.y);
+      acc.y += cf * (fr * v.y + fi * v.x);
+    }
+    return make_double2(-acc.x, -acc.y);
+  };
+  const double2 zero = make_double2(0.0, 0.0);
+  if (M00) M00[idx] = blk(jp0 - i0);
+  if (M10) M10[idx] = (jp0 >= i0) ? blk(jp0 - i0 - s) : zero;
+  if (M01) M01[idx] = (jp0 <= i0) ? blk(jp0 - i0 + s) : zero;
+}
+
+static int ensure_clebsch(skkr_ctx* c) {
+  if (c->dU1) return SKKR_OK;
+  std::vector<double> u1, u2;
+  std::vector<int> i1, i2;
+  kkrh::clebsch_tables(u1, u2, i1, i2);
+  CHK(dalloc_t(c, &c->dU1, u1.size()));
+  CHK(dalloc_t(c, &c->dU2, u2.size()));
+  CHK(dalloc_t(c, &c->dI1, i1.size()));
+  CHK(dalloc_t(c, &c->dI2, i2.size()));
+  HIPCHK(hipMemcpy(c->dU1, u1.data(), sizeof(double) * u1.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dU2, u2.data(), sizeof(double) * u2.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dI1, i1.data(), sizeof(int) * i1.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dI2, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice));
+  return SKKR_OK;
+}
+
+static int ensure_g2d(skkr_ctx* c, size_t count) {
+  if (count <= c->g2d_cap) return SKKR_OK;
+  if (c->dG2d) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->dG2d);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    HIPCHK(hipFree(c->dG2d));
+    c->dG2d = nullptr;
+  }
+  CHK(dalloc_t(c, &c->dG2d, count));
+  c->g2d_cap = count;
+  return SKKR_OK;
+}
+
+static int launch_g2d_pack(skkr_ctx* c, int irel, int nl2, const G2dShape& sh, const z* win, double fr,
+                           double fi, z* m00, z* m10, z* m01) {
+  const int s = c->cfg.sites_per_pl, lk = c->cfg.lk;
+  const long long ne = (long long)c->nn;
+  g2d_pack_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, c->stream>>>(
+      s, lk, nl2, sh, win, irel, fr, fi, c->dU1, c->dU2, c->dI1, c->dI2, m00, m10, m01);
+  HIPCHK(hipGetLastError());
+  c->launches++;
+  return SKKR_OK;
+}
+
+// common checks of the reference-shaped entries: the basis, the window and
+// the padded dimensions against the context.  irel = 0 reads lkmax x lkmax
+// blocks of g2d and never lmax (packer.f passes it to relmtrx only); irel = 1
+// builds kappa-mu blocks of 2 (lmax+1)^2 from (lmax+1)^2 lm blocks.
+static int check_ref_shape(skkr_ctx* c, int irel, int lmax, int lkmax, int lkmaxp, int mdimp, const G2dShape& sh,
+                           int* nl2) {
+  if (irel != 0 && irel != 1) return SKKR_ERR_ARG;
+  if (lkmax != c->cfg.lk || lkmaxp < lkmax || mdimp < c->n || sh.minprc < c->cfg.sites_per_pl) return SKKR_ERR_ARG;
+  if (irel) {
+    if (lmax < 0 || lmax > 5 || lkmax != 2 * (lmax + 1) * (lmax + 1)) return SKKR_ERR_ARG;
+    *nl2 = (lmax + 1) * (lmax + 1);
+  } else {
+    *nl2 = lkmax;
+  }
+  if (sh.lmmaxp < *nl2) return SKKR_ERR_ARG;
+  return SKKR_OK;
+}
+
+// lk x lk blocks of a padded (lkmaxp, lkmaxp, *) array -> contiguous
+static void unpad_blocks(const double* src, int lkmaxp, int lk, int nblk, std::vector<double>& dst) {
+  dst.assign(2 * (size_t)lk * lk * nblk, 0.0);
+  for (int b = 0; b < nblk; ++b)
+    for (int cc = 0; cc < lk; ++cc)
+      memcpy(&dst[2 * ((size_t)b * lk * lk + (size_t)cc * lk)],
+             src + 2 * ((size_t)b * lkmaxp * lkmaxp + (size_t)cc * lkmaxp), sizeof(z) * lk);
+}
+
+int skkr_tau2d_g2d(skkr_ctx* c, int irel, double wgeff, int bulkgeo, int lmax, int lkmax, int lkmaxp,
+                   int nintfc, const double* tminv, const double* g2d, const double* x, const double* y,
+                   double* tau, int mdimp, int lmmaxp, int minprc, int mprc1, int mintfc, int nprc,
+                   const int* ninprc, double fac_re, double fac_im) {
+  if (!c || !tminv || !g2d || !x || !y || !tau || !ninprc) return SKKR_ERR_ARG;
+  const G2dShape sh{lmmaxp, minprc};
+  int nl2 = 0;
+  CHK(check_ref_shape(c, irel, lmax, lkmax, lkmaxp, mdimp, sh, &nl2));
+  const int P = c->cfg.n_pl, s = c->cfg.sites_per_pl, lk = c->cfg.lk, n = c->n;
+  if (nprc != P || nintfc != P * s || mintfc < nintfc || mprc1 < 0) return SKKR_ERR_ARG;
+  if (!bulkgeo && nprc > mprc1 + 1) return SKKR_ERR_ARG;  // planes 1..nprc of 0:mprc1+1
+  for (int p = 1; p <= nprc; ++p)
+    if (ninprc[p] != s) return SKKR_ERR_UNSUPPORTED;  // equal principal layers only
+  HIPCHK(hipSetDevice(c->cfg.device));
+  CHK(ensure_batch(c, std::max(c->B, 1)));
+  if (!bulkgeo) CHK(ensure_layered(c));
+  if (irel) CHK(ensure_clebsch(c));
+  const size_t W = sh.window(), nn = c->nn, Bs = c->B;
+  const int nplanes = bulkgeo ? 1 : nprc;
+  CHK(ensure_g2d(c, W * nplanes));
+  // planes 1..nplanes of g2d(.., 0:mprc1+1)
+  HIPCHK(hipMemcpyAsync(c->dG2d, g2d + 2 * W, sizeof(z) * W * nplanes, hipMemcpyHostToDevice, c->stream));
+  if (bulkgeo) {
+    CHK(launch_g2d_pack(c, irel, nl2, sh, c->dG2d, fac_re, fac_im, c->M00, c->M10, c->M01));
+  } else {
+    for (int p = 0; p < nprc; ++p)
+      CHK(launch_g2d_pack(c, irel, nl2, sh, c->dG2d + (size_t)p * W, fac_re, fac_im,
+                          c->M00pl + (size_t)p * Bs * nn, c->M10pl + (size_t)p * Bs * nn,
+                          c->M01pl + (size_t)p * Bs * nn));
+  }
+  // the interface t^-1 (replaces the context's, as skkr_set_tinv would)
+  std::vector<double> ht;
+  unpad_blocks(tminv, lkmaxp, lk, nintfc, ht);
+  HIPCHK(hipMemcpyAsync(c->dTint, ht.data(), sizeof(double) * ht.size(), hipMemcpyHostToDevice, c->stream));
+  // x, y: leading n x n of mdimp x mdimp
+  std::vector<double> hy;
+  unpad_blocks(y, mdimp, n, 1, hy);
+  HIPCHK(hipMemcpy2DAsync(c->Xc, sizeof(z) * n, x, sizeof(z) * mdimp, sizeof(z) * n, n, hipMemcpyHostToDevice,
+                          c->stream));
+  HIPCHK(hipMemcpyAsync(c->Ycur, hy.data(), sizeof(z) * nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->status_sw, 0, sizeof(int), c->stream));
+  c->layered = !bulkgeo;
+  const int rs = sweeps_checked(c, 1, hy.data());
+  c->layered = false;
+  CHK(rs);
+  const size_t tsz = (size_t)nintfc * lk * lk;
+  scale_copy_kernel<<<(unsigned)((tsz + 255) / 256), 256, 0, c->stream>>>((long long)tsz, c->tauk, wgeff, c->tauk);
+  HIPCHK(hipGetLastError());
+  std::vector<double> ho(2 * tsz);
+  int hs = 0;
+  HIPCHK(hipMemcpyAsync(ho.data(), c->tauk, sizeof(z) * tsz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&hs, c->status_sw, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // tau(l, lp, ilay) of the padded (lkmaxp, lkmaxp, mintfc) array
+  for (int b = 0; b < nintfc; ++b)
+    for (int cc = 0; cc < lk; ++cc)
+      memcpy(tau + 2 * ((size_t)b * lkmaxp * lkmaxp + (size_t)cc * lkmaxp),
+             &ho[2 * ((size_t)b * lk * lk + (size_t)cc * lk)], sizeof(z) * lk);
+  return status_code(hs);
+}
+
+int skkr_dugo_g2d(skkr_ctx* c, int lmax, int lkmax, const double* tminvb, int lkmaxp, const double* g2d,
+                  const double* g2di, double* xy, int mdimp, char side, int bulk, int irel, int* sgfconv,
+                  int lmmaxp, int minprc, int nprc, const int* ninprc, double fac_re, double fac_im,
+                  int* iters) {
+  if (!c || !tminvb || !g2d || !g2di || !xy || !ninprc || (side != 'L' && side != 'R')) return SKKR_ERR_ARG;
+  const G2dShape sh{lmmaxp, minprc};
+  int nl2 = 0;
+  CHK(check_ref_shape(c, irel, lmax, lkmax, lkmaxp, mdimp, sh, &nl2));
+  const int s = c->cfg.sites_per_pl, lk = c->cfg.lk, n = c->n;
+  if (nprc < 1) return SKKR_ERR_ARG;
+  // the lead PL and the interface PL next to it (dugo.f:44-72)
+  const int ncpl = bulk ? ninprc[1] : (side == 'L' ? ninprc[0] : ninprc[nprc + 1]);
+  const int ncpli = bulk ? ncpl : (side == 'L' ? ninprc[1] : ninprc[nprc]);
+  if (ncpl != s || ncpli != s) return SKKR_ERR_UNSUPPORTED;  // equal principal layers only
+  HIPCHK(hipSetDevice(c->cfg.device));
+  CHK(ensure_batch(c, std::max(c->B, 1)));
+  if (irel) CHK(ensure_clebsch(c));
+  const size_t W = sh.window(), nn = c->nn;
+  const int B = c->B;
+  CHK(ensure_g2d(c, 2 * W));
+  if (!c->dMt) CHK(dalloc_t(c, &c->dMt, nn));
+  if (c->dmt_cap != 2 * B) {
+    CHK(make_ptrs(c, &c->p_dmt, std::vector<z*>(2 * B, c->dMt)));
+    c->dmt_cap = 2 * B;
+  }
+  HIPCHK(hipMemcpyAsync(c->dG2d, g2d, sizeof(z) * W, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dG2d + W, g2di, sizeof(z) * W, hipMemcpyHostToDevice, c->stream));
+  const z* wg = c->dG2d;
+  const z* wgi = c->dG2d + W;
+  // bulk: the lead's blocks from g2di, the extra factor mt from g2d; otherwise
+  // the lead's blocks from g2d and mt, the interface PL's coupling to the lead,
+  // from g2di.  Chain 0 (left, ml = M10, mr = M01) or B (right, ml = M01, mr = M10).
+  const z* wdec = bulk ? wgi : wg;
+  const z* wmt = bulk ? wg : wgi;
+  CHK(launch_g2d_pack(c, irel, nl2, sh, wdec, fac_re, fac_im, c->M00, c->M10, c->M01));
+  const bool mt_is_m01 = (side == 'L') == (bool)bulk;  // bulk L, non-bulk R: mt = packer's m01
+  CHK(launch_g2d_pack(c, irel, nl2, sh, wmt, fac_re, fac_im, nullptr, mt_is_m01 ? nullptr : c->dMt,
+                      mt_is_m01 ? c->dMt : nullptr));
+  // the lead t^-1 of this side
+  std::vector<double> ht;
+  unpad_blocks(tminvb, lkmaxp, lk, s, ht);
+  HIPCHK(hipMemcpyAsync(side == 'L' ? c->dTl : c->dTr, ht.data(), sizeof(double) * ht.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  // xy = m10 g mt (bulk L), mt g m01 (L), m01 g mt (bulk R), mt g m10 (R)
+  CHK(dec_device(c, 1, true, side, true, bulk ? &c->p_dmt : nullptr, bulk ? nullptr : &c->p_dmt));
+  std::vector<double> hxy(2 * nn);
+  HIPCHK(hipMemcpyAsync(hxy.data(), side == 'R' ? c->Xc : c->Ycur, sizeof(z) * nn, hipMemcpyDeviceToHost,
+                        c->stream));
+  int it = 0, st = 0;
+  const int rc = dec_results(c, 1, side, &it, &st);
+  for (int cc = 0; cc < n; ++cc)
+    memcpy(xy + 2 * (size_t)cc * mdimp, &hxy[2 * (size_t)cc * n], sizeof(z) * n);
+  if (iters) *iters = it;
+  if (sgfconv) *sgfconv = (st == SKKR_ERR_SGF_DIVERGED) ? 0 : 1;
   return rc;
 }
 

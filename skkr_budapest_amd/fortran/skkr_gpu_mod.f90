@@ -7,7 +7,8 @@ module skkr_gpu
 
   integer(c_int), parameter :: SKKR_OK = 0, SKKR_ERR_ARG = 1, SKKR_ERR_HIP = 2, &
        SKKR_ERR_NOMEM = 3, SKKR_ERR_SINGULAR = 4, SKKR_ERR_SGF_DIVERGED = 5, &
-       SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7, SKKR_ERR_STR_NOCONV = 8
+       SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7, SKKR_ERR_STR_NOCONV = 8, &
+       SKKR_ERR_TOO_LARGE = 9
   integer(c_int), parameter :: SKKR_FP64 = 0, SKKR_MIXED_FP32 = 1, SKKR_MIXED_BF16 = 2
 
   type, bind(C) :: skkr_config
@@ -208,6 +209,41 @@ module skkr_gpu
        complex(c_double_complex), intent(out) :: tau(*)
        integer(c_int), intent(out) :: status(*)
      end function skkr_tau2d_layered_batch
+
+     ! tau2d (main/tau2d.f:2-3) with the reference's own arrays, one k-point;
+     ! then param.h's lmmaxp, minprc, mprc1, mintfc, common /lay2d/'s nprc,
+     ! ninprc(0:nprc+1) and common /relfac/'s fac
+     integer(c_int) function skkr_tau2d_g2d(ctx, irel, wgeff, bulkgeo, lmax, lkmax, lkmaxp, nintfc, &
+          tminv, g2d, x, y, tau, mdimp, lmmaxp, minprc, mprc1, mintfc, nprc, ninprc, fac_re, fac_im) &
+          bind(C, name="skkr_tau2d_g2d")
+       import :: c_int, c_ptr, c_double, c_double_complex
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: irel, bulkgeo, lmax, lkmax, lkmaxp, nintfc, mdimp
+       real(c_double), value :: wgeff
+       complex(c_double_complex), intent(in) :: tminv(*), g2d(*), x(*), y(*)
+       complex(c_double_complex), intent(inout) :: tau(*)
+       integer(c_int), value :: lmmaxp, minprc, mprc1, mintfc, nprc
+       integer(c_int), intent(in) :: ninprc(*)
+       real(c_double), value :: fac_re, fac_im
+     end function skkr_tau2d_g2d
+
+     ! dugo (main/dugo.f:9-10) with the reference's own arrays, one k-point;
+     ! then lmmaxp, minprc, nprc, ninprc(0:nprc+1), fac, iters (out)
+     integer(c_int) function skkr_dugo_g2d(ctx, lmax, lkmax, tminvb, lkmaxp, g2d, g2di, xy, mdimp, &
+          side, bulk, irel, sgfconv, lmmaxp, minprc, nprc, ninprc, fac_re, fac_im, iters) &
+          bind(C, name="skkr_dugo_g2d")
+       import :: c_int, c_ptr, c_double, c_double_complex, c_char
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: lmax, lkmax, lkmaxp, mdimp, bulk, irel
+       complex(c_double_complex), intent(in) :: tminvb(*), g2d(*), g2di(*)
+       complex(c_double_complex), intent(inout) :: xy(*)
+       character(kind=c_char), value :: side
+       integer(c_int), intent(out) :: sgfconv
+       integer(c_int), value :: lmmaxp, minprc, nprc
+       integer(c_int), intent(in) :: ninprc(*)
+       real(c_double), value :: fac_re, fac_im
+       integer(c_int), intent(out) :: iters
+     end function skkr_dugo_g2d
 
      ! point group of an irreducible-wedge mesh (sphbas dg, dg1; cpacoord.f:294-299)
      integer(c_int) function skkr_set_point_group(ctx, ng, dg, dg1) bind(C, name="skkr_set_point_group")

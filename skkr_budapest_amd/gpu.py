@@ -37,7 +37,7 @@ SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structu
            "skkr_tau_bz_phys", "skkr_str_create", "skkr_str_destroy", "skkr_gstore_batch",
            "skkr_set_structure_kkr", "skkr_cpa_bz", "skkr_set_pairs", "skkr_tau_bz_pairs",
            "skkr_set_structure_kkr_rel", "skkr_tau2d_layered_batch", "skkr_set_point_group",
-           "skkr_surfgfd_noalloc")
+           "skkr_surfgfd_noalloc", "skkr_tau2d_g2d", "skkr_dugo_g2d")
 
 
 # codes that name a failing k-point or matrix (reported per item in status
@@ -129,6 +129,8 @@ def lib():
         L.skkr_set_pairs.argtypes = [V, I, V, V, V]
         L.skkr_tau_bz_pairs.argtypes = [V, D, D, I, V, V, V, V, V, V, V, ctypes.POINTER(Stats)]
         L.skkr_surfgfd_noalloc.argtypes = [I, I, I, V, V, V, V, V, V, D, I, V]
+        L.skkr_tau2d_g2d.argtypes = [V, I, D, I, I, I, I, I, V, V, V, V, V, I, I, I, I, I, I, V, D, D]
+        L.skkr_dugo_g2d.argtypes = [V, I, I, V, I, V, V, V, I, ctypes.c_char, I, I, V, I, I, I, V, D, D, V]
         _LIB = L
     return _LIB
 
@@ -425,6 +427,65 @@ class Context:
         rc = lib().skkr_tau2d_layered_batch(self._h, nb, *map(_p, bufs), float(w), _p(tb), _p(status))
         self._check_call(rc, "skkr_tau2d_layered_batch")
         return from_colmajor(tb), status
+
+    # -- the reference's own argument lists (one k-point per call) ----------
+    @staticmethod
+    def _fortran(a, shape):
+        a = np.asfortranarray(a, np.complex128)
+        if a.shape != tuple(shape):
+            raise ValueError(f"array of shape {a.shape}, expected {tuple(shape)}")
+        return a
+
+    def tau2d_g2d(self, irel, wgeff, bulkgeo, lmax, tminv, g2d, x, y, tau, ninprc, fac=1.0):
+        """tau2d(irel, wgeff, bulkgeo, lmax, lkmax, lkmaxp, nintfc, tminv, g2d,
+        x, y, tau, mdimp) of skkr_kpar_v6/main/tau2d.f:2-3 on Fortran-ordered
+        arrays: tminv and tau (lkmaxp, lkmaxp, mintfc), g2d (lmmaxp, lmmaxp,
+        2 minprc + 1, minprc, mprc1 + 2), x and y (mdimp, mdimp); ninprc is
+        common /lay2d/'s ninprc(0:nprc+1).  tau is written in place (its first
+        nintfc layers).  Returns the status code."""
+        lkmaxp, _, mintfc = tminv.shape
+        lmmaxp, _, nj, minprc, npl = g2d.shape
+        mdimp = x.shape[0]
+        ninprc = np.ascontiguousarray(ninprc, np.int32)
+        nprc = len(ninprc) - 2
+        nintfc = int(ninprc[1:nprc + 1].sum())
+        tm = self._fortran(tminv, (lkmaxp, lkmaxp, mintfc))
+        g = self._fortran(g2d, (lmmaxp, lmmaxp, 2 * minprc + 1, minprc, npl))
+        xx, yy = self._fortran(x, (mdimp, mdimp)), self._fortran(y, (mdimp, mdimp))
+        if not (isinstance(tau, np.ndarray) and tau.dtype == np.complex128 and tau.flags.f_contiguous
+                and tau.shape == (lkmaxp, lkmaxp, mintfc)):
+            raise ValueError("tau must be a Fortran-ordered complex128 (lkmaxp, lkmaxp, mintfc) array")
+        rc = lib().skkr_tau2d_g2d(self._h, irel, float(wgeff), int(bool(bulkgeo)), lmax, self.lk, lkmaxp, nintfc,
+                                  _p(tm), _p(g), _p(xx), _p(yy), _p(tau), mdimp, lmmaxp, minprc, npl - 2,
+                                  mintfc, nprc, _p(ninprc), complex(fac).real, complex(fac).imag)
+        self._check_call(rc, "skkr_tau2d_g2d")
+        return rc
+
+    def dugo_g2d(self, lmax, tminvb, g2d, g2di, xy, side, bulk, irel, ninprc, fac=1.0):
+        """dugo(lmax, lkmax, tminvb, lkmaxp, g2d, g2di, xy, mdimp, side, bulk,
+        irel, sgfconv) of skkr_kpar_v6/main/dugo.f:9-10 on Fortran-ordered
+        arrays: tminvb (lkmaxp, lkmaxp, minprc), g2d and g2di (lmmaxp, lmmaxp,
+        2 minprc + 1, minprc), xy (mdimp, mdimp), written in place.  Returns
+        (status code, sgfconv, iterations)."""
+        lkmaxp = tminvb.shape[0]
+        lmmaxp, _, nj, minprc = g2d.shape
+        mdimp = xy.shape[0]
+        ninprc = np.ascontiguousarray(ninprc, np.int32)
+        nprc = len(ninprc) - 2
+        tb = self._fortran(tminvb, (lkmaxp, lkmaxp, tminvb.shape[2]))
+        g = self._fortran(g2d, (lmmaxp, lmmaxp, 2 * minprc + 1, minprc))
+        gi = self._fortran(g2di, (lmmaxp, lmmaxp, 2 * minprc + 1, minprc))
+        if not (isinstance(xy, np.ndarray) and xy.dtype == np.complex128 and xy.flags.f_contiguous
+                and xy.shape == (mdimp, mdimp)):
+            raise ValueError("xy must be a Fortran-ordered complex128 (mdimp, mdimp) array")
+        conv = ctypes.c_int(0)
+        its = ctypes.c_int(0)
+        rc = lib().skkr_dugo_g2d(self._h, lmax, self.lk, _p(tb), lkmaxp, _p(g), _p(gi), _p(xy), mdimp,
+                                 side.encode() if isinstance(side, str) else side, int(bool(bulk)), irel,
+                                 ctypes.byref(conv), lmmaxp, minprc, nprc, _p(ninprc),
+                                 complex(fac).real, complex(fac).imag, ctypes.byref(its))
+        self._check_call(rc, "skkr_dugo_g2d")
+        return rc, bool(conv.value), its.value
 
 
 class Structure:

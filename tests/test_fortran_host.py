@@ -141,3 +141,58 @@ def test_fortran_tau2d_layered_matches_reference_golden(tmp_path):
     for i in range(3):
         ref, _ = oracle.tau2d(b00[i], b01[i], b10[i], ti, sy.cfg.P, bx[i], by[i], w=0.5)
         assert np.linalg.norm(tau[i] - ref) < 1e-11 * np.linalg.norm(ref)
+
+
+G2D_EXE = os.path.join(ROOT, "skkr_budapest_amd", "fortran", "skkr_g2d")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def write_g2d_input(path, sy, E, k, w, isave):
+    """The stream the reference driver that wrote the golden vectors reads
+    (lk, s, P, nk, itermax, nsave; errmax; isave; t^-1 of interface and
+    leads; w; per k G_intra(k), G_inter(k))."""
+    c = sy.cfg
+    with open(path, "wb") as f:
+        np.array([c.lk, c.s, c.P, len(w), c.itermax, len(isave)], np.int32).tofile(f)
+        np.array([c.errmax], np.float64).tofile(f)
+        np.array([i + 1 for i in isave], np.int32).tofile(f)
+        for t in sy.tinv(E):
+            S.colmajor(t).tofile(f)
+        np.ascontiguousarray(w, np.float64).tofile(f)
+        for kk in k:
+            ph = np.exp(-1j * (sy.R @ kk))
+            S.colmajor(np.tensordot(ph, sy.G_intra, axes=(0, 0))).tofile(f)
+            S.colmajor(np.tensordot(ph, sy.G_inter, axes=(0, 0))).tofile(f)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["cfg0_spin1", "herm16"])
+def test_fortran_reference_shaped_dugo_tau2d_match_golden(tmp_path, name):
+    """A Fortran k-loop calling dugo_gpu and tau2d_gpu with the argument lists
+    of dugo.f:9-10 and tau2d.f:2-3 (g2d windows, padded arrays) against the
+    reference's own golden vectors: BZ sum, per-k traces, the saved per-k tau
+    and the decimation iteration counts."""
+    from test_oracle import GOLDEN
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    sy = GOLDEN[name]
+    c = sy.cfg
+    E = complex(g["E"])
+    ns, lk, nk = c.P * c.s, c.lk, len(g["w"])
+    isave = list(g["save_idx"])
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_g2d_input(fin, sy, E, g["k"], g["w"], isave)
+    print(f"fortran g2d host: {name}", flush=True)
+    subprocess.run([G2D_EXE, str(fin), str(fout)], check=True, timeout=120)
+    raw = np.fromfile(fout, np.complex128)
+    o = 0
+    taubz = S.from_colmajor(raw[o:o + ns * lk * lk].reshape(ns, lk, lk)); o += ns * lk * lk
+    trk = raw[o:o + ns * nk].reshape(nk, ns); o += ns * nk
+    tsave = S.from_colmajor(raw[o:o + len(isave) * ns * lk * lk].reshape(len(isave), ns, lk, lk))
+    o += len(isave) * ns * lk * lk
+    iters = np.frombuffer(raw[o:].tobytes(), np.int32).reshape(nk, 2)
+    np.testing.assert_array_equal(iters, g["iters"])
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(taubz, g["tau_bz"]) < 1e-12
+    assert np.abs(trk - g["trk"]).max() < 1e-12 * np.abs(g["trk"]).max()
+    for j in range(len(isave)):
+        assert rel(tsave[j], g["tau_save"][j]) < 1e-12
