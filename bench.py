@@ -241,7 +241,7 @@ def kkr_sample_bench(args):
     4.667470441, beta = 0.7071), lmax 2 relativistic (kappa-mu, lk 18), the
     16 contour energies of input_rsp.in:32 (zmesh) with the k-sets of
     input_rsp.in:35-39 (intbz 1 wedges of kmesh2d: 253 ... 1953 k) and the C2v
-    group sum of the reference's rsp sphbas.  t^-1 and screening are square
+    group sum of the reference's rsp sphbas (the matrices from pointgroup).  t^-1 and screening are square
     wells (the sample's potentials are not used)."""
     from skkr_budapest_amd import contour, gpu
     from skkr_budapest_amd import kkrsystems as K
@@ -251,7 +251,8 @@ def kkr_sample_bench(args):
     st = K.nb110_workload(z[0], P=3, lmax=2, rel=1)["structure"]
     lat = KM.primkr2d(2, "C2vx", st["a1"][0], 0.70710678120, 0.0)
     meshes = {s: KM.kmesh2d(1, s, lat) for s in sorted(set(ksets))}
-    sph = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "sphbas_rel.npz"))
+    from skkr_budapest_amd import pointgroup as PG
+    dg, dg1, _ = PG.group_matrices("C2v", lmax=2, rel=True)
     nk_total = sum(len(meshes[s][1]) for s in ksets)
     print(f"bench: sample 02-layer-scf energy loop, 16 energies, {nk_total} k-evaluations", file=sys.stderr,
           flush=True)
@@ -269,7 +270,7 @@ def kkr_sample_bench(args):
         return time.perf_counter() - t0, r
 
     with gpu.Structure(**st) as sc, gpu.Context(lk=18, sites_per_pl=st["s"], n_pl=3) as ctx:
-        ctx.set_point_group(sph["C2v_dg"], sph["C2v_dg1"])
+        ctx.set_point_group(dg, dg1)
         for _ in range(args.warmup):
             one_energy(ctx, sc, 0)
             one_energy(ctx, sc, 15)
