@@ -7,9 +7,10 @@ both leads, Godfrin sweeps, BZ accumulation), inputs resident in HBM.
 Default workload: the relativistic layer system with lk=32, 4 sites per
 principal layer (n=128), 20 interface PLs, 25 in-plane vectors, one energy
 E=0.6+0.02i Ry and a 64x64 k-mesh (4096 k-points), timed first in FP64 and
-then in the split-BF16 mixed mode; `value` is the mixed mode's k-points/s,
-`config` holds both rates, their ratio and the mixed result's error against
-the FP64 one (tau_BZ, tr G, DOS).  --fp64-only times FP64 alone.
+then in the two mixed modes (INT8 digit products, split-BF16 products);
+`value` is the INT8 mixed mode's k-points/s, `config` holds every rate, the
+ratios to FP64 and each mixed result's error against the FP64 one (tau_BZ,
+tr G, DOS).  --fp64-only times FP64 alone.
 
 --kkr runs the k-loop on the reference's own structure constants instead of
 the synthetic Fourier sum: the Nb(110) stack of sample/02-layer-scf (lmax 2,
@@ -38,6 +39,7 @@ METRIC = ("BZ-integrated tau k-points/s per energy: mixed+refine vs FP64, "
           "at 1e-8 rel err")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0  # dense BF16 MFMA, MI355X datasheet
+PEAK_INT8_TOPS = 5000.0    # dense INT8 MFMA, MI355X datasheet
 # HBM bytes moved per algorithmic byte by zgemm_f64_kernel<128,128,16>: rocprofv3
 # FETCH_SIZE + WRITE_SIZE of 1024 x complex 128^3 products, calibrated with a
 # plain 256 MiB copy (profiles/hbm_traffic_pmc.md): 805.8 MB / 805.3 MB
@@ -72,7 +74,12 @@ def cpu_baseline(sy, E, k, w):
 
 
 def contour_bench(args, cfg, sy):
+    """A step is the whole contour.  Config 2 times FP64 and, unless
+    --fp64-only, the two mixed modes; `value` is the INT8 mixed mode's rate and
+    every energy reports the mixed results' errors against FP64 (tau_BZ,
+    tr G, DOS).  Config 3 (BdG, n = 256) times FP64."""
     from skkr_budapest_amd import contour, energy_loop, gpu
+    from skkr_budapest_amd import observables as O
     z, wz = contour.config_contour()
     if args.config == 2:
         sides = [cfg.mesh] * len(z)
@@ -81,32 +88,55 @@ def contour_bench(args, cfg, sy):
     nk_total = sum(s * s for s in sides)
     print(f"bench: config {args.config} contour {len(z)} energies, {nk_total} k in total",
           file=sys.stderr, flush=True)
-    ctx = gpu.Context.for_system(sy)
-    for i in range(args.warmup):
-        energy_loop.run_contour(ctx, sy, z[-1:], wz[-1:], sides[-1:])
-    times = []
-    recs = None
-    for i in range(args.steps):
-        t0 = time.perf_counter()
-        recs, total = energy_loop.run_contour(ctx, sy, z, wz, sides)
-        times.append(time.perf_counter() - t0)
-        print(f"bench: contour step {i} {times[-1]:.2f} s, integral of tr G {total}",
-              file=sys.stderr, flush=True)
-    ctx.close()
-    sec = float(np.mean(times))
+    modes = [("fp64", gpu.FP64)]
+    if args.config == 2 and not args.fp64_only:
+        modes += [("mixed_int8", gpu.MIXED_INT8), ("mixed_bf16", gpu.MIXED_BF16)]
+    runs = {}
+    for name, prec in modes:
+        ctx = gpu.Context.for_system(sy, precision=prec)
+        for i in range(args.warmup):
+            energy_loop.run_contour(ctx, sy, z[-1:], wz[-1:], sides[-1:])
+        times = []
+        recs = total = None
+        for i in range(args.steps):
+            t0 = time.perf_counter()
+            recs, total = energy_loop.run_contour(ctx, sy, z, wz, sides)
+            times.append(time.perf_counter() - t0)
+            print(f"bench: {name} contour step {i} {times[-1]:.2f} s, integral of tr G {total}",
+                  file=sys.stderr, flush=True)
+        ctx.close()
+        runs[name] = {"sec": float(np.mean(times)), "recs": recs, "total": total}
+    head_name = "mixed_int8" if "mixed_int8" in runs else "fp64"
+    head = runs[head_name]
+    fp = runs["fp64"]
+    energies = []
+    for j, r in enumerate(head["recs"]):
+        e = {"E": [r["E"].real, r["E"].imag], "nk": r["nk"], "seconds": r["seconds"],
+             "dos": r["dos"], "mean_iter": float(r["iters"].mean())}
+        for name in runs:
+            if name != "fp64":
+                e[f"{name}_error_vs_fp64"] = O.error_report(runs[name]["recs"][j]["tau_bz"],
+                                                            fp["recs"][j]["tau_bz"])
+        energies.append(e)
+    cfg_out = {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} n={cfg.n} "
+                           f"N_R={cfg.n_R}, 16-point contour E_b=-0.8 E_F=0.6, "
+                           f"{nk_total} k in total",
+               "precision": head_name,
+               "integrated_trG": [head["total"].real, head["total"].imag],
+               "energies": energies}
+    for name, r in runs.items():
+        cfg_out[name] = {"k_points_per_s": nk_total / r["sec"], "seconds_per_contour": r["sec"]}
+        if name != "fp64":
+            cfg_out[name]["over_fp64"] = fp["sec"] / r["sec"]
+            cfg_out[name]["integrated_trG_rel_err"] = abs(r["total"] - fp["total"]) / abs(fp["total"])
+            cfg_out[name]["max_tau_rel_err"] = max(e[f"{name}_error_vs_fp64"]["tau_rel_err"]
+                                                   for e in energies)
     out = {
-        "metric": METRIC, "value": nk_total / sec, "unit": "k-points/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * sec,
+        "metric": METRIC, "value": nk_total / head["sec"], "unit": "k-points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * head["sec"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic (seeded G_r(R), t^-1)",
-        "config": {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} n={cfg.n} "
-                               f"N_R={cfg.n_R}, 16-point contour E_b=-0.8 E_F=0.6, "
-                               f"{nk_total} k in total",
-                   "precision": "fp64",
-                   "integrated_trG": [total.real, total.imag],
-                   "energies": [{"E": [r["E"].real, r["E"].imag], "nk": r["nk"],
-                                 "seconds": r["seconds"], "dos": r["dos"],
-                                 "mean_iter": float(r["iters"].mean())} for r in recs]},
+        "config": cfg_out,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -311,7 +341,10 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true",
-                    help="time only the FP64 k-loop (default: FP64, then the split-BF16 mixed mode)")
+                    help="time only the FP64 k-loop (default: FP64, then the INT8 and split-BF16 mixed modes)")
+    ap.add_argument("--modes", default="",
+                    help="comma list of k-loop modes to time (fp64, mixed_int8, mixed_bf16); "
+                         "default all (fp64 alone with --fp64-only)")
     ap.add_argument("--kkr", action="store_true",
                     help="k-loop on the layer structure-constant producer (Nb(110) stack)")
     ap.add_argument("--kkr-sample", action="store_true",
@@ -345,7 +378,11 @@ def main():
     print(f"bench: config {args.config} n={cfg.n} P={cfg.P} nk={nk} E={E}", file=sys.stderr,
           flush=True)
 
-    modes = [("fp64", gpu.FP64)] + ([] if args.fp64_only else [("mixed_bf16", gpu.MIXED_BF16)])
+    modes = [("fp64", gpu.FP64)] + ([] if args.fp64_only else [("mixed_int8", gpu.MIXED_INT8),
+                                                                ("mixed_bf16", gpu.MIXED_BF16)])
+    if args.modes:
+        want = args.modes.split(",")
+        modes = [md for md in modes if md[0] in want]
     runs = {}
     for name, prec in modes:
         ctx = gpu.Context.for_system(sy, precision=prec)
@@ -355,17 +392,21 @@ def main():
         ctx.set_tinv(*sy.tinv(E))
         runs[name] = kloop_steps(ctx, E, k, w, args, name)
         ctx.close()
-    head = runs["mixed_bf16"] if "mixed_bf16" in runs else runs["fp64"]
+    head_name = "mixed_int8" if "mixed_int8" in runs else list(runs)[0]
+    head = runs[head_name]
     value = nk / (head["ms"] * 1e-3)
     cfg_out = {"workload": f"cfg{args.config}: lk={cfg.lk} s={cfg.s} P={cfg.P} "
                            f"n={cfg.n} N_R={cfg.n_R} E={E} {cfg.mesh}x{cfg.mesh} k",
-               "precision": "mixed_bf16" if "mixed_bf16" in runs else "fp64", "nk": nk}
+               "precision": head_name, "nk": nk}
     for name, r in runs.items():
         cfg_out[name] = mode_summary(r, nk, cfg)
-    if "mixed_bf16" in runs:
+    if "mixed_int8" in runs and "fp64" in runs and "mixed_bf16" in runs:
         from skkr_budapest_amd import observables as O
-        cfg_out["mixed_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_bf16"]["ms"]
-        cfg_out["mixed_error_vs_fp64"] = O.error_report(runs["mixed_bf16"]["tau_bz"], runs["fp64"]["tau_bz"])
+        cfg_out["mixed_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_int8"]["ms"]
+        cfg_out["mixed_error_vs_fp64"] = O.error_report(runs["mixed_int8"]["tau_bz"], runs["fp64"]["tau_bz"])
+        cfg_out["mixed_bf16_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_bf16"]["ms"]
+        cfg_out["mixed_bf16_error_vs_fp64"] = O.error_report(runs["mixed_bf16"]["tau_bz"],
+                                                             runs["fp64"]["tau_bz"])
     rl = rooflines(head)
     out = {
         "metric": METRIC,
@@ -430,7 +471,14 @@ def rooflines(run):
             continue
         alg = flops / (ms * 1e-3) / 1e12
         per = flops / launches
-        if cls == "dom" and stats[-1]["dom_name"].startswith("zgemm_split"):
+        if cls == "dom" and stats[-1]["dom_name"].startswith("zgemm_i8"):
+            exe = sum(s["dom_flops_exec"] for s in stats) / (ms * 1e-3) / 1e12
+            ent = {"bound": "mfma", "kernel": "zgemm_i8_kernel", "achieved": exe,
+                   "peak": PEAK_INT8_TOPS, "unit": "TOP/s", "frac": exe / PEAK_INT8_TOPS,
+                   "achieved_is": "executed INT8 MFMA ops",
+                   "algorithmic_tflops_fp64_equivalent": alg,
+                   "algorithmic_frac_of_fp64_peak": alg / PEAK_FP64_TFLOPS}
+        elif cls == "dom" and stats[-1]["dom_name"].startswith("zgemm_split"):
             exe = sum(s["dom_flops_exec"] for s in stats) / (ms * 1e-3) / 1e12
             ent = {"bound": "mfma", "kernel": "zgemm_split_kernel", "achieved": exe,
                    "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": exe / PEAK_BF16_TFLOPS,

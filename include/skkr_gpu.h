@@ -48,7 +48,7 @@ enum {
   SKKR_FP64 = 0,        /* every product and inversion in FP64 */
   SKKR_MIXED_FP32 = 1,  /* not built: FP32 products lose ~5 digits in the
                            decimation (SKKR_ERR_UNSUPPORTED) */
-  SKKR_MIXED_BF16 = 2   /* split-BF16 products: every complex GEMM of the
+  SKKR_MIXED_BF16 = 2,  /* split-BF16 products: every complex GEMM of the
                            decimation and the PL sweeps of skkr_tau_bz* as
                            exact BF16 slice products (Ozaki scheme) summed in
                            FP32 and recombined in FP64 -- S = 5 slices per
@@ -57,6 +57,12 @@ enum {
                            sum in FP64; for Im E < fp64_dec_below_imE the
                            decimation runs in FP64.  The per-routine entry
                            points stay FP64. */
+  SKKR_MIXED_INT8 = 3   /* INT8 digit products: the same GEMMs as exact
+                           products of S = 6 balanced base-256 digits per
+                           operand value (2^-47 of its row / column maximum
+                           per K-chunk of 32) on the INT8 MFMA, summed in
+                           int32 and recombined in FP64 per chunk; the rest
+                           as SKKR_MIXED_BF16 */
 };
 
 typedef struct skkr_ctx skkr_ctx;
@@ -68,7 +74,8 @@ typedef struct {
   double errmax;       /* decimation stop |alpha|_F < errmax (/iterpar/) */
   int itermax_l;       /* itermaxl */
   int itermax_r;       /* itermaxr */
-  int precision;       /* SKKR_FP64 | SKKR_MIXED_FP32 | SKKR_MIXED_BF16 */
+  int precision;       /* SKKR_FP64 | SKKR_MIXED_BF16 | SKKR_MIXED_INT8
+                          (SKKR_MIXED_FP32: SKKR_ERR_UNSUPPORTED) */
   double fp64_dec_below_imE; /* mixed modes: FP64 decimation when Im E < this */
   int device;          /* HIP device ordinal */
   int batch;           /* k-points per launch group; 0 = automatic */
@@ -94,7 +101,9 @@ typedef struct {
   char dom_name[32];
   double dom_bytes;       /* algorithmic HBM bytes of those launches (A, B read, C written) */
   double dom_flops_exec;  /* SKKR_MIXED_BF16: BF16 MFMA flops the split products
-                             executed (slice pairs x 4 real products, padded tiles) */
+                             executed (slice pairs x 4 real products, padded
+                             tiles); SKKR_MIXED_INT8: INT8 MFMA ops (digit
+                             pairs x 4 real products, padded tiles) */
   /* inversion launches (gjinv kernels), timed like the GEMMs; in skkr_tau_bz*
    * the flops and bytes of both classes count only the decimation chains that
    * were still iterating (n <= 128) */
@@ -176,9 +185,12 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *   "split_dec", "split_sw", "split_sw_low"  SKKR_MIXED_BF16: slices per
  *               operand (2..5, or 0 for FP64 products) of the decimation, of
  *               the sweeps, and of the sweeps below fp64_dec_below_imE
- *               (defaults 5, 5, 5)
+ *               (defaults 5, 5, 5); SKKR_MIXED_INT8: digits per operand
+ *               (2..6, or 0; defaults 6, 6, 6)
  *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
- *               that many slices (0: FP64) */
+ *               that many slices (0: FP64)
+ *   "zgemm_int8" 0 | 2..6: skkr_zgemm_batch runs the INT8 digit product with
+ *               that many digits (0: FP64) */
 int skkr_set_option(skkr_ctx *ctx, const char *key, int value);
 
 typedef struct {

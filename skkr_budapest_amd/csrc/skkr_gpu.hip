@@ -24,6 +24,7 @@
 #include "kernels_f64.h"
 #include "gjinv_blk.h"
 #include "zgemm_split.h"
+#include "zgemm_i8.h"
 #include "kkrlat.h"
 #include "kkrstr.h"
 #include "cpa.h"
@@ -132,14 +133,17 @@ struct skkr_ctx {
   int tile_nn = 0, tile_2nn = 0;  // autotuned tiles (0 = by shape)
   int graphs = 1;  // sweeps of a batch replayed from a captured graph (bitwise equal to launches)
   bool capturing = false;
-  // split-BF16 products (SKKR_MIXED_BF16): slices of the GEMMs being launched
-  // (0 = FP64 MFMA kernel), and the choices per phase
+  // split products (SKKR_MIXED_BF16 / SKKR_MIXED_INT8): slices (BF16) or
+  // digits (INT8) of the GEMMs being launched (0 = FP64 MFMA kernel), the
+  // kind, and the choices per phase
   int split = 0;
+  int split_i8 = 0;       // 1: INT8 digit products (zgemm_i8.h), 0: BF16 slices
   int split_dec = 5;      // decimation and lead self-energies, Im E >= fp64_dec_below_imE
   int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
   int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
-  int zgemm_split = 0;    // skkr_zgemm_batch: 0 FP64, else slices
-  double dom_flops_exec = 0.0;  // executed BF16 MFMA flops of the split launches
+  int zgemm_split = 0;    // skkr_zgemm_batch: 0 FP64, else slices or digits
+  int zgemm_i8 = 0;       // skkr_zgemm_batch: the product kind (1 INT8 digits)
+  double dom_flops_exec = 0.0;  // executed BF16 MFMA flops (INT8 MFMA ops) of the split launches
   // screening of the interface sites (physical tau, phystau.f)
   bool have_screening = false;
   int nl = 0;
@@ -438,12 +442,43 @@ static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
   *tiles = ((m + t - 1) / t) * ((n + tn - 1) / tn);
 }
 
-// Split-BF16 product (zgemm_split.h) with S = c->split slices per operand.
+// Split-BF16 product (zgemm_split.h) with S = c->split slices per operand,
+// or with c->split_i8 the INT8 digit product (zgemm_i8.h) with S digits.
 static int split_tiles(int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); }
+static int i8_tiles(int m, int n) { return ((m + 127) / 128) * ((n + 63) / 64); }
+static int split_tiles(const skkr_ctx* c, int m, int n) { return c->split_i8 ? i8_tiles(m, n) : split_tiles(m, n); }
+
+static int launch_zgemm_i8(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
+                           z** C, int ldc, z alpha, z beta, int nb, const int* active,
+                           const EpiArgs& epi) {
+  dim3 grid(i8_tiles(m, n), nb);
+  dom_begin(c);
+  const z* const* a = (const z* const*)A;
+  const z* const* b = (const z* const*)B;
+  switch (c->split) {
+    case 2: zgemm_i8_kernel<2><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 3: zgemm_i8_kernel<3><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 4: zgemm_i8_kernel<4><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 5: zgemm_i8_kernel<5><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    case 6: zgemm_i8_kernel<6><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+    default: return SKKR_ERR_UNSUPPORTED;
+  }
+  HIPCHK(hipGetLastError());
+  // executed INT8 MFMA work: S (S + 1) / 2 digit pairs x 4 real products on
+  // the padded 128 x 64 x 32 tiles
+  const double pm = (m + 127) / 128 * 128.0, pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
+  const double pairs = c->split * (c->split + 1) / 2.0;
+  if (c->timing_detail && !c->capturing) c->dom_flops_exec += pairs * 4.0 * 2.0 * pm * pn * pk * nb;
+  dom_end(c, 8.0 * m * n * (double)k * nb,
+          16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
+  c->launches++;
+  return SKKR_OK;
+}
 
 static int launch_zgemm_split(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
                               z** C, int ldc, z alpha, z beta, int nb, const int* active,
                               const EpiArgs& epi) {
+  if (c->split_i8) return launch_zgemm_i8(c, m, n, k, A, lda, B, ldb, C, ldc, alpha, beta, nb, active, epi);
   dim3 grid(split_tiles(m, n), nb);
   dom_begin(c);
   const z* const* a = (const z* const*)A;
@@ -679,7 +714,7 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   int tm, tiles;
   gemm_tile(c, 2 * n, n, &tm, &tiles);
-  if (c->split > 0) tiles = split_tiles(2 * n, n);  // work-groups of the split kernel
+  if (c->split > 0) tiles = split_tiles(c, 2 * n, n);  // work-groups of the split kernel
   int cur = 0;
   c->dec_rounds = 0;
   // Iterations are launched in rounds without waiting: the first round as many
@@ -825,7 +860,7 @@ static int sweep_launches(skkr_ctx* c, int nb) {
 // buffers and pointer arrays are fixed for the context's batch size).
 static int run_sweeps(skkr_ctx* c, int nb) {
   if (!c->graphs || c->pairs_on || c->layered) return sweep_launches(c, nb);
-  const int key = 8 * nb + c->split;
+  const int key = 8 * nb + c->split + 1000000 * c->split_i8;
   auto it = c->sweep_graphs.find(key);
   if (it == c->sweep_graphs.end()) {
     hipGraph_t g = nullptr;
@@ -885,13 +920,19 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   if (cfg->lk <= 0 || cfg->sites_per_pl <= 0 || cfg->n_pl <= 0 || cfg->errmax <= 0 ||
       cfg->itermax_l < 1 || cfg->itermax_r < 1)
     return SKKR_ERR_ARG;
-  if (cfg->precision != SKKR_FP64 && cfg->precision != SKKR_MIXED_BF16) return SKKR_ERR_UNSUPPORTED;
+  if (cfg->precision != SKKR_FP64 && cfg->precision != SKKR_MIXED_BF16 && cfg->precision != SKKR_MIXED_INT8)
+    return SKKR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
     return SKKR_ERR_HIP;
   HIPCHK(hipSetDevice(cfg->device));
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
+  if (cfg->precision == SKKR_MIXED_INT8) {
+    // 6 digits (2^-47 of the line maximum per operand value) in every phase
+    c->split_i8 = 1;
+    c->split_dec = c->split_sw = c->split_sw_low = 6;
+  }
   if (const char* e = getenv("SKKR_INV_KERNEL"))
     c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
   if (const char* e = getenv("SKKR_GEMM_3M")) c->gemm_3m = (atoi(e) != 0) ? 1 : 0;
@@ -1164,7 +1205,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   // mixed precision (SKKR_MIXED_BF16): split-BF16 products in the decimation
   // and the sweeps; the decimation stays FP64 for Im E below the threshold
   int s_dec = 0, s_sw = 0;
-  if (c->cfg.precision == SKKR_MIXED_BF16) {
+  if (c->cfg.precision == SKKR_MIXED_BF16 || c->cfg.precision == SKKR_MIXED_INT8) {
     const bool near_axis = c->e_im < c->cfg.fp64_dec_below_imE;
     s_dec = near_axis ? 0 : c->split_dec;
     s_sw = near_axis ? c->split_sw_low : c->split_sw;
@@ -1304,9 +1345,10 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
       const double started = 2.0 * nb;
       const double corr = (it_sum - rounds * B2d) * 3.0 * 16.0 * n3 + (started - B2d) * 2.0 * 8.0 * n3;
       c->dom_flops += corr;
-      if (s_dec > 0) {  // executed split work of the same launches (64 x 64 x 32 padding)
-        const double pad = (double)((2 * c->n + 63) / 64 * 64) * ((c->n + 63) / 64 * 64) * ((c->n + 31) / 32 * 32) /
-                           (2.0 * n3);
+      if (s_dec > 0) {  // executed split work of the same launches (padded tiles)
+        const int tm = c->split_i8 ? 128 : 64;
+        const double pad = (double)((2 * c->n + tm - 1) / tm * tm) * ((c->n + 63) / 64 * 64) *
+                           ((c->n + 31) / 32 * 32) / (2.0 * n3);
         c->dom_flops_exec += corr * s_dec * (s_dec + 1) / 2.0 * pad;
       }
       c->dom_bytes += (it_sum - rounds * B2d) * 3.0 * 80.0 * n2 + (started - B2d) * 2.0 * 48.0 * n2;
@@ -1385,7 +1427,8 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     st->inv_bytes = c->inv_bytes;
     st->redo_batches = c->redo_batches;
     snprintf(st->dom_name, sizeof(st->dom_name), "%s",
-             (s_dec > 0 && s_sw > 0) ? "zgemm_split_kernel" : (s_sw > 0 ? "zgemm_f64+split" : "zgemm_f64_kernel"));
+             (s_dec > 0 && s_sw > 0) ? (c->split_i8 ? "zgemm_i8_kernel" : "zgemm_split_kernel")
+                                     : (s_sw > 0 ? "zgemm_f64+split" : "zgemm_f64_kernel"));
   }
   c->timing_detail = 0;
   return rc;
@@ -1452,13 +1495,21 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     drop_graphs(c);
     return SKKR_OK;
   }
-  if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0 ||
-      strcmp(key, "zgemm_split") == 0) {
-    if (value != 0 && (value < 2 || value > 5)) return SKKR_ERR_ARG;
+  if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0) {
+    // BF16 slices 2..5, INT8 digits 2..6 (by the context's precision)
+    const int smax = c->split_i8 ? 6 : 5;
+    if (value != 0 && (value < 2 || value > smax)) return SKKR_ERR_ARG;
     if (strcmp(key, "split_dec") == 0) c->split_dec = value;
     else if (strcmp(key, "split_sw") == 0) c->split_sw = value;
-    else if (strcmp(key, "split_sw_low") == 0) c->split_sw_low = value;
-    else c->zgemm_split = value;
+    else c->split_sw_low = value;
+    drop_graphs(c);
+    return SKKR_OK;
+  }
+  if (strcmp(key, "zgemm_split") == 0 || strcmp(key, "zgemm_int8") == 0) {
+    const bool i8 = strcmp(key, "zgemm_int8") == 0;
+    if (value != 0 && (value < 2 || value > (i8 ? 6 : 5))) return SKKR_ERR_ARG;
+    c->zgemm_split = value;
+    c->zgemm_i8 = i8 ? 1 : 0;
     return SKKR_OK;
   }
   if (strcmp(key, "gemm_3m") == 0) {
@@ -1639,6 +1690,12 @@ int skkr_zgemm_batch(skkr_ctx* c, int n, int nb, const double* a, const double* 
   CHK(Bm.init(nb, nn, b, c->stream));
   CHK(C.init(nb, nn, nullptr, c->stream));
   SplitScope sp(c, c->zgemm_split);
+  struct KindScope {
+    skkr_ctx* c;
+    int saved;
+    ~KindScope() { c->split_i8 = saved; }
+  } kind{c, c->split_i8};
+  c->split_i8 = c->zgemm_i8;
   CHK(launch_zgemm(c, n, n, n, A.p, n, Bm.p, n, C.p, n, make_double2(1, 0), make_double2(0, 0), nb, nullptr));
   HIPCHK(hipMemcpyAsync(cc, C.d, sizeof(z) * nn * nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

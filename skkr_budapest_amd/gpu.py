@@ -27,7 +27,7 @@ ERRORS = {1: "SKKR_ERR_ARG", 2: "SKKR_ERR_HIP", 3: "SKKR_ERR_NOMEM",
           4: "SKKR_ERR_SINGULAR", 5: "SKKR_ERR_SGF_DIVERGED",
           6: "SKKR_ERR_SGF_ITERMAX", 7: "SKKR_ERR_UNSUPPORTED", 8: "SKKR_ERR_STR_NOCONV",
           9: "SKKR_ERR_TOO_LARGE"}
-FP64, MIXED_FP32, MIXED_BF16 = 0, 1, 2
+FP64, MIXED_FP32, MIXED_BF16, MIXED_INT8 = 0, 1, 2, 3
 
 # exported symbols (every declaration of include/skkr_gpu.h)
 SYMBOLS = ("skkr_create", "skkr_destroy", "skkr_error_string", "skkr_set_structure",

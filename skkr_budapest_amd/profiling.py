@@ -4,6 +4,7 @@
   python -m skkr_budapest_amd.profiling gemm     # 1024 complex GEMMs 128^3
   python -m skkr_budapest_amd.profiling inverse  # 1024 inversions 128 x 128
   python -m skkr_budapest_amd.profiling split    # the same GEMMs: FP64, then split-BF16 with 2..5 slices
+  python -m skkr_budapest_amd.profiling int8     # the same GEMMs: FP64, then INT8 digits, 4..6 per operand
 
 The copy moves a known number of bytes, so its FETCH_SIZE / WRITE_SIZE
 counters calibrate those of the other kernels.  Each workload runs 3 times.
@@ -54,6 +55,17 @@ def run_split(n=128, nb=1024, slices=(0, 2, 3, 4, 5)):
             print(f"split: {nb} x complex {n}^3 with {s} slices per operand (0: FP64 kernel)", flush=True)
 
 
+def run_int8(n=128, nb=1024, digits=(0, 4, 5, 6)):
+    from . import gpu
+    a, b = _mats(n, nb, 1), _mats(n, nb, 2)
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        for s in digits:
+            ctx.set_option("zgemm_int8", s)
+            for _ in range(3):
+                ctx.zgemm(a, b)
+            print(f"int8: {nb} x complex {n}^3 with {s} digits per operand (0: FP64 kernel)", flush=True)
+
+
 def run_inverse(n=128, nb=1024):
     from . import gpu
     h = _mats(n, nb, 3)
@@ -67,4 +79,5 @@ def run_inverse(n=128, nb=1024):
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "gemm"
-    {"copy": run_copy, "gemm": run_gemm, "inverse": run_inverse, "split": run_split}[what]()
+    {"copy": run_copy, "gemm": run_gemm, "inverse": run_inverse, "split": run_split,
+     "int8": run_int8}[what]()

@@ -8,6 +8,7 @@ result of the same inputs -- the accuracy the mixed mode is built for."""
 import numpy as np
 import pytest
 
+import i8_emul as ie
 import oracle
 import split_emul as se
 from skkr_budapest_amd import contour
@@ -48,22 +49,49 @@ def test_split_gemm_matches_emulation_bitwise(gpu, n):
     assert np.linalg.norm(c64 - a @ b) < 1e-14 * np.linalg.norm(a @ b)
 
 
+@pytest.mark.parametrize("n", [36, 128, 200])
+def test_i8_gemm_matches_emulation_bitwise(gpu, n):
+    """The INT8 digit product for every digit count on ragged shapes (n = 36,
+    200: partial 128 x 64 tiles and K-chunks) and a batch of 11; rows and
+    columns scaled over 12 orders of magnitude."""
+    rng = np.random.default_rng(2000 + n)
+    nb = 11
+    a = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    b = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
+    a *= np.logspace(-6, 6, n)[None, :, None]
+    b *= np.logspace(4, -8, n)[None, None, :]
+    print(f"INT8 GEMM n={n} nb={nb}", flush=True)
+    with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        for s in (2, 3, 4, 5, 6):
+            ctx.set_option("zgemm_int8", s)
+            c = ctx.zgemm(a, b)
+            for j in range(nb):
+                np.testing.assert_array_equal(c[j], ie.i8_product(a[j], b[j], s))
+        with pytest.raises(gpu.SkkrError):
+            ctx.set_option("zgemm_int8", 7)
+        ctx.set_option("zgemm_int8", 0)
+        c64 = ctx.zgemm(a, b)
+    assert np.linalg.norm(c64 - a @ b) < 1e-14 * np.linalg.norm(a @ b)
+
+
 def _errors(tau, ref):
     return O.error_report(tau, ref)
 
 
-def test_mixed_config1_sample_matches_oracle(gpu):
+@pytest.mark.parametrize("mode", ["MIXED_BF16", "MIXED_INT8"])
+def test_mixed_config1_sample_matches_oracle(gpu, mode):
     """Config 1 (n = 128, 20 PLs, E = 0.6 + 0.02i) on 32 k-points of its
-    64 x 64 mesh: the mixed k-loop against the FP64 oracle."""
+    64 x 64 mesh: the mixed k-loop (both product kinds) against the FP64
+    oracle."""
     cfg = S.CONFIGS[1]
     sy = S.make_system(cfg)
     E = cfg.energies[0]
     k, w = S.uniform_kmesh(cfg.mesh)
     idx = np.linspace(0, len(w) - 1, 32).astype(int)
     ks, ws = k[idx], np.full(len(idx), 1.0 / len(idx))
-    print("mixed cfg1 sample: 32 k", flush=True)
+    print(f"mixed {mode} cfg1 sample: 32 k", flush=True)
     ref = oracle.tau_bz(sy, E, ks, ws, nworkers=16)
-    with gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as ctx:
+    with gpu.Context.for_system(sy, precision=getattr(gpu, mode)) as ctx:
         ctx.set_tinv(*sy.tinv(E))
         r = ctx.tau_bz(E, ks, ws)
     assert (r["status"] == 0).all()
@@ -80,36 +108,41 @@ def test_mixed_config1_sample_matches_oracle(gpu):
 
 def test_mixed_contour_config2(gpu):
     """Config 2: the config-1 system on the 16-point contour (Im E from 0.695
-    down to 0.005), full 64 x 64 mesh per energy, mixed against FP64 on the
-    GPU, with the oracle on 2 k-points per energy.  Below Im E = 0.02 the
-    decimation runs in FP64 (fp64_dec_below_imE)."""
+    down to 0.005), full 64 x 64 mesh per energy, both mixed modes (split-BF16
+    and INT8 digits) against FP64 on the GPU, with the oracle on 2 k-points
+    per energy.  Below Im E = 0.02 the decimation runs in FP64
+    (fp64_dec_below_imE)."""
     cfg = S.CONFIGS[2]
     sy = S.make_system(cfg)
     z, wz = contour.config_contour()
     k, w = S.uniform_kmesh(cfg.mesh)
-    fp, mx = [], []
-    with gpu.Context.for_system(sy) as c64, gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as cmx:
-        for ie, E in enumerate(z):
-            print(f"contour energy {ie} E={E:.4f}", flush=True)
-            for c, out in ((c64, fp), (cmx, mx)):
+    fp, mx = [], {"bf16": [], "int8": []}
+    with gpu.Context.for_system(sy) as c64, \
+            gpu.Context.for_system(sy, precision=gpu.MIXED_BF16) as cbf, \
+            gpu.Context.for_system(sy, precision=gpu.MIXED_INT8) as ci8:
+        for je, E in enumerate(z):
+            print(f"contour energy {je} E={E:.4f}", flush=True)
+            for c, out in ((c64, fp), (cbf, mx["bf16"]), (ci8, mx["int8"])):
                 c.set_tinv(*sy.tinv(E))
                 r = c.tau_bz(E, k, w)
                 assert (r["status"] == 0).all()
                 out.append(r)
-            e = _errors(mx[-1]["tau_bz"], fp[-1]["tau_bz"])
-            print("  mixed vs FP64", e, mx[-1]["stats"]["dom_name"], flush=True)
-            for v in e.values():
-                assert v < MIXED_TOL
-            # near the axis the decimation is FP64
-            if E.imag < 0.02:
-                np.testing.assert_array_equal(mx[-1]["iters"], fp[-1]["iters"])
-            # oracle spot check of the per-k traces
             sel = np.array([7, 2900])
             ref = oracle.tau_bz(sy, E, k[sel], w[sel], nworkers=2)
-            assert np.abs(mx[-1]["trk"][sel] - ref["trk"]).max() < MIXED_TOL * np.abs(ref["trk"]).max()
+            for name, rs in mx.items():
+                e = _errors(rs[-1]["tau_bz"], fp[-1]["tau_bz"])
+                print(f"  {name} vs FP64", e, rs[-1]["stats"]["dom_name"], flush=True)
+                for v in e.values():
+                    assert v < MIXED_TOL
+                # near the axis the decimation is FP64
+                if E.imag < 0.02:
+                    np.testing.assert_array_equal(rs[-1]["iters"], fp[-1]["iters"])
+                # oracle spot check of the per-k traces
+                assert np.abs(rs[-1]["trk"][sel] - ref["trk"]).max() < MIXED_TOL * np.abs(ref["trk"]).max()
     g64 = O.integrated_g([O.tr_g(r["tau_bz"]) for r in fp], wz)
-    gmx = O.integrated_g([O.tr_g(r["tau_bz"]) for r in mx], wz)
-    assert abs(gmx - g64) < MIXED_TOL * abs(g64)
+    for rs in mx.values():
+        gmx = O.integrated_g([O.tr_g(r["tau_bz"]) for r in rs], wz)
+        assert abs(gmx - g64) < MIXED_TOL * abs(g64)
 
 
 def test_mixed_precision_choices(gpu):
@@ -138,3 +171,21 @@ def test_mixed_precision_choices(gpu):
     assert e_def < MIXED_TOL and e3 < 1e-4 and e_def < e3 / 100
     with pytest.raises(gpu.SkkrError):
         gpu.Context(lk=8, sites_per_pl=2, n_pl=4, precision=gpu.MIXED_FP32)
+    # INT8 digits: 2 .. 6 per operand, 6 by default
+    with gpu.Context.for_system(sy, precision=gpu.MIXED_INT8) as c:
+        c.set_tinv(*sy.tinv(E))
+        d6 = c.tau_bz(E, k, w)
+        c.set_option("split_dec", 3)
+        c.set_option("split_sw", 3)
+        d3 = c.tau_bz(E, k, w)
+        c.set_option("split_dec", 0)
+        c.set_option("split_sw", 0)
+        d0 = c.tau_bz(E, k, w)
+        with pytest.raises(gpu.SkkrError):
+            c.set_option("split_sw", 7)
+    np.testing.assert_array_equal(d0["tau_bz"], ref["tau_bz"])
+    f6 = O.rel_err(d6["tau_bz"], ref["tau_bz"])
+    f3 = O.rel_err(d3["tau_bz"], ref["tau_bz"])
+    print("INT8 default", f6, "three digits", f3, flush=True)
+    # three digits hold 22 bits (three BF16 slices 26)
+    assert f6 < MIXED_TOL and f3 < 1e-3 and f6 < f3 / 100
