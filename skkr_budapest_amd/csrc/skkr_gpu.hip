@@ -154,6 +154,7 @@ struct skkr_ctx {
   z* phk = nullptr;      // per-k per-site per-l traces of a batch
   int phk_cap = 0;
   std::map<int, hipGraphExec_t> sweep_graphs;  // keyed by 8 nb + split
+  std::map<long long, hipGraphExec_t> dec_graphs;  // decimation rounds, keyed by iterations and products
   // structure constants from the layer producer instead of the Fourier sum
   skkr_str* str = nullptr;
   z* strAlpha = nullptr;          // (lmax+1) x s screening parameters
@@ -245,6 +246,8 @@ static int free_batch(skkr_ctx* c) {
 static void drop_graphs(skkr_ctx* c) {
   for (auto& kv : c->sweep_graphs) hipGraphExecDestroy(kv.second);
   c->sweep_graphs.clear();
+  for (auto& kv : c->dec_graphs) hipGraphExecDestroy(kv.second);
+  c->dec_graphs.clear();
 }
 
 static int ensure_batch(skkr_ctx* c, int B) {
@@ -351,6 +354,7 @@ static int ensure_batch(skkr_ctx* c, int B) {
 static int ensure_inverse(skkr_ctx* c, int B) {
   const int n = c->n;
   if (n <= 128 || c->invB >= B) return SKKR_OK;
+  drop_graphs(c);  // captured launches hold the previous buffers
   const size_t nn = c->nn, B2 = 2 * (size_t)B, h1 = 128, h2 = n - 128;
   CHK(dalloc_bt(c, &c->Asc, B2 * nn));
   CHK(dalloc_bt(c, &c->Csc, B2 * h1 * h2));
@@ -694,6 +698,79 @@ static int run_fourier(skkr_ctx* c, int nk) {
   return SKKR_OK;
 }
 
+// Decimation iterations first .. last of all 2B chains (surfgf.f:47-96): per
+// iteration y = eps^-1, [ay; by] = [alpha; beta] y, then [ay; by] beta and
+// [ay; by] alpha with the updates of eps, x and the next [alpha; beta] fused
+// into the GEMM epilogues, and the per-chain stop test.  cur: the buffer of
+// [alpha; beta] that iteration `first` reads.
+static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) {
+  const int n = c->n, B2 = 2 * c->B;
+  const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  for (int iter = first; iter <= last; ++iter) {
+    InvArgs ia = inv_args(c);
+    ia.base = c->p_eps.d;
+    ia.out = c->p_Yinv.d;
+    ia.active = c->active;
+    ia.status = c->status;
+    CHK(run_inverse(c, ia, B2));
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
+                     one, zero, B2, c->active));
+    EpiArgs e;
+    memset(&e, 0, sizeof(e));
+    e.n = n;
+    e.eps = c->p_eps.d;
+    e.x = c->p_x.d;
+    e.abn = c->p_AB[cur ^ 1].d;
+    e.norm = c->dnorm;
+    e.tiles = tiles;
+    e.mode = EPI_DEC_BETA;
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
+                     one, zero, B2, c->active, &e));
+    e.mode = EPI_DEC_ALPHA;
+    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
+                     one, zero, B2, c->active, &e));
+    dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
+        B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
+        c->active, c->iters, c->status, c->nactive + iter);
+    HIPCHK(hipGetLastError());
+    c->launches++;
+    cur ^= 1;
+  }
+  return SKKR_OK;
+}
+
+// One round of decimation iterations: replayed from a captured HIP graph
+// (one per first / last iteration and product kind; the kernels and their
+// arguments are those of the launches) unless per-launch timing is asked for.
+static int dec_round(skkr_ctx* c, int first, int last, int cur, int tiles) {
+  if (!c->graphs || c->capturing || c->timing_detail) return dec_iterations(c, first, last, cur, tiles);
+  const long long key = (((long long)first * 256 + last) * 16 + c->split) * 2 + c->split_i8;
+  auto it = c->dec_graphs.find(key);
+  if (it == c->dec_graphs.end()) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    const long long l0 = c->launches;
+    const int r = dec_iterations(c, first, last, cur, tiles);
+    c->launches = l0;
+    c->capturing = false;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r != SKKR_OK) {
+      if (g) hipGraphDestroy(g);
+      return r;
+    }
+    HIPCHK(e);
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HIPCHK(ei);
+    it = c->dec_graphs.emplace(key, ge).first;
+  }
+  HIPCHK(hipGraphLaunch(it->second, c->stream));
+  c->launches++;  // one replay counts as one launch
+  return SKKR_OK;
+}
+
 // Decimation of chains j in [0, 2B) whose active flag the caller set; chains
 // use p_ml/p_mr/p_m0/p_tl.  Per iteration (surfgf.f:47-96): y = eps^-1,
 // [ay; by] = [alpha; beta] y, then [ay; by] beta and [ay; by] alpha with the
@@ -725,37 +802,10 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
   int iter = 1, round = std::max(2, std::min(c->dec_hint, itermax + 1));
   while (iter <= itermax + 1) {
     const int last = std::min(itermax + 1, iter + round - 1);
-    for (; iter <= last; ++iter) {
-      c->dec_rounds = iter;
-      InvArgs ia = inv_args(c);
-      ia.base = c->p_eps.d;
-      ia.out = c->p_Yinv.d;
-      ia.active = c->active;
-      ia.status = c->status;
-      CHK(run_inverse(c, ia, B2));
-      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
-                       one, zero, B2, c->active));
-      EpiArgs e;
-      memset(&e, 0, sizeof(e));
-      e.n = n;
-      e.eps = c->p_eps.d;
-      e.x = c->p_x.d;
-      e.abn = c->p_AB[cur ^ 1].d;
-      e.norm = c->dnorm;
-      e.tiles = tiles;
-      e.mode = EPI_DEC_BETA;
-      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
-                       one, zero, B2, c->active, &e));
-      e.mode = EPI_DEC_ALPHA;
-      CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
-                       one, zero, B2, c->active, &e));
-      dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
-          B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
-          c->active, c->iters, c->status, c->nactive + iter);
-      HIPCHK(hipGetLastError());
-      c->launches++;
-      cur ^= 1;
-    }
+    CHK(dec_round(c, iter, last, cur, tiles));
+    cur ^= (last - iter + 1) & 1;
+    iter = last + 1;
+    c->dec_rounds = last;
     int nact = 0;
     HIPCHK(hipMemcpyAsync(&nact, c->nactive + (iter - 1), sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

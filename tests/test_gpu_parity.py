@@ -352,8 +352,9 @@ def test_config3_sample_matches_oracle(gpu):
 
 
 def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
-    """HIP-graph replays of the sweeps, forced GEMM tiles and the autotuned
-    choice all give the same FP64 bits as the eager default."""
+    """HIP-graph replays of the decimation rounds and the sweeps, forced GEMM
+    tiles and the autotuned choice all give the same FP64 bits (and decimation
+    iteration counts) as the eager default."""
     sy = small_sys(lk=16, s=4, P=6, r2max=4, E=0.4 + 0.05j)
     E = sy.cfg.energies[0]
     k, w = S.uniform_kmesh(6)
@@ -366,6 +367,8 @@ def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
             r = ctx.tau_bz(E, k, w)
             np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
             np.testing.assert_array_equal(r["trk"], base["trk"])
+            np.testing.assert_array_equal(r["iters"], base["iters"])
+            assert r["stats"]["n_launches"] < base["stats"]["n_launches"]
             # eager work on another batch size between replays
             ctx.set_option("graphs", 0)
             ctx.tau_bz(E, k[:7], w[:7])
@@ -380,6 +383,24 @@ def test_graph_replay_and_tiles_are_bitwise_identical(gpu):
         assert t["tile_nn"] in (32, 64, 128) and t["tile_2nn"] in (32, 64, 128)
         r = ctx.tau_bz(E, k, w)
         np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+
+
+def test_graph_replay_mixed_int8_bitwise(gpu):
+    """The INT8 mixed mode replays its decimation rounds and sweeps from graphs
+    with the bits of its eager launches, at two energies in turn (different
+    round lengths)."""
+    sy = small_sys(lk=16, s=4, P=6, r2max=4, E=0.4 + 0.05j)
+    k, w = S.uniform_kmesh(6)
+    with gpu.Context.for_system(sy, precision=gpu.MIXED_INT8) as ctx:
+        for E in (0.4 + 0.05j, 0.1 + 0.3j, 0.4 + 0.05j):
+            ctx.set_tinv(*sy.tinv(E))
+            ctx.set_option("graphs", 0)
+            base = ctx.tau_bz(E, k, w)
+            ctx.set_option("graphs", 1)
+            for rep in range(2):
+                r = ctx.tau_bz(E, k, w)
+                np.testing.assert_array_equal(r["tau_bz"], base["tau_bz"])
+                np.testing.assert_array_equal(r["iters"], base["iters"])
 
 
 def test_smaller_call_after_a_larger_batch(gpu):
