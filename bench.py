@@ -383,11 +383,14 @@ def main():
     if args.modes:
         want = args.modes.split(",")
         modes = [md for md in modes if md[0] in want]
-    runs = {}
+    # the modes' timed steps alternate (fp64, int8, bf16, fp64, ...) so that
+    # each sees the same clock and temperature of the device
+    ctxs = {}
     for name, prec in modes:
-        ctx = gpu.Context.for_system(sy, precision=prec)
-        ctx.set_tinv(*sy.tinv(E))
-        runs[name] = kloop_steps(ctx, E, k, w, args, name)
+        ctxs[name] = gpu.Context.for_system(sy, precision=prec)
+        ctxs[name].set_tinv(*sy.tinv(E))
+    runs = kloop_steps(ctxs, E, k, w, args)
+    for ctx in ctxs.values():
         ctx.close()
     head_name = "mixed_int8" if "mixed_int8" in runs else list(runs)[0]
     head = runs[head_name]
@@ -428,28 +431,37 @@ def main():
     print(json.dumps(out))
 
 
-def kloop_steps(ctx, E, k, w, args, name):
+def kloop_steps(ctxs, E, k, w, args):
     """Timed steps on the default path (decimation rounds and sweeps replayed
-    from HIP graphs), then one step with eager launches, each bracketed by HIP
-    events, for the per-kernel figures (the replays hide their launches)."""
-    for i in range(args.warmup):
-        r = ctx.tau_bz(E, k, w)
-        print(f"bench: {name} warmup {i} {r['stats']['ms_total']:.1f} ms", file=sys.stderr, flush=True)
-    times, stats = [], []
+    from HIP graphs), the modes taking turns step by step; then one step per
+    mode with eager launches, each bracketed by HIP events, for the per-kernel
+    figures (the replays hide their launches)."""
+    for name, ctx in ctxs.items():
+        for i in range(args.warmup):
+            r = ctx.tau_bz(E, k, w)
+            print(f"bench: {name} warmup {i} {r['stats']['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    times = {name: [] for name in ctxs}
+    stats = {name: [] for name in ctxs}
+    last = {}
     for i in range(args.steps):
-        t0 = time.perf_counter()
-        r = ctx.tau_bz(E, k, w)
-        times.append(time.perf_counter() - t0)
-        stats.append(r["stats"])
-        print(f"bench: {name} step {i} {times[-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr,
+        for name, ctx in ctxs.items():
+            t0 = time.perf_counter()
+            r = ctx.tau_bz(E, k, w)
+            times[name].append(time.perf_counter() - t0)
+            stats[name].append(r["stats"])
+            last[name] = r
+            print(f"bench: {name} step {i} {times[name][-1] * 1e3:.1f} ms {r['stats']}", file=sys.stderr,
+                  flush=True)
+    runs = {}
+    for name, ctx in ctxs.items():
+        ctx.set_option("graphs", 0)
+        d = ctx.tau_bz(E, k, w, timing_detail=True)
+        ctx.set_option("graphs", 1)
+        print(f"bench: {name} detail step {d['stats']['ms_total']:.1f} ms {d['stats']}", file=sys.stderr,
               flush=True)
-    ctx.set_option("graphs", 0)
-    d = ctx.tau_bz(E, k, w, timing_detail=True)
-    ctx.set_option("graphs", 1)
-    print(f"bench: {name} detail step {d['stats']['ms_total']:.1f} ms {d['stats']}", file=sys.stderr,
-          flush=True)
-    return {"ms": 1e3 * float(np.mean(times)), "stats": stats, "detail": [d["stats"]],
-            "tau_bz": r["tau_bz"], "ms_eager": d["stats"]["ms_total"]}
+        runs[name] = {"ms": 1e3 * float(np.mean(times[name])), "stats": stats[name], "detail": [d["stats"]],
+                      "tau_bz": last[name]["tau_bz"], "ms_eager": d["stats"]["ms_total"]}
+    return runs
 
 
 def mode_summary(r, nk, cfg):
