@@ -77,7 +77,8 @@ def contour_bench(args, cfg, sy):
     """A step is the whole contour.  Config 2 times FP64 and, unless
     --fp64-only, the two mixed modes; `value` is the INT8 mixed mode's rate and
     every energy reports the mixed results' errors against FP64 (tau_BZ,
-    tr G, DOS).  Config 3 (BdG, n = 256) times FP64."""
+    tr G, DOS).  Config 3 (BdG, n = 256) times FP64, or the modes --modes
+    names."""
     from skkr_budapest_amd import contour, energy_loop, gpu
     from skkr_budapest_amd import observables as O
     z, wz = contour.config_contour()
@@ -89,8 +90,11 @@ def contour_bench(args, cfg, sy):
     print(f"bench: config {args.config} contour {len(z)} energies, {nk_total} k in total",
           file=sys.stderr, flush=True)
     modes = [("fp64", gpu.FP64)]
-    if args.config == 2 and not args.fp64_only:
+    if (args.config == 2 or args.modes) and not args.fp64_only:
         modes += [("mixed_int8", gpu.MIXED_INT8), ("mixed_bf16", gpu.MIXED_BF16)]
+    if args.modes:
+        want = args.modes.split(",")
+        modes = [md for md in modes if md[0] in want]
     runs = {}
     for name, prec in modes:
         ctx = gpu.Context.for_system(sy, precision=prec)
@@ -108,13 +112,13 @@ def contour_bench(args, cfg, sy):
         runs[name] = {"sec": float(np.mean(times)), "recs": recs, "total": total}
     head_name = "mixed_int8" if "mixed_int8" in runs else "fp64"
     head = runs[head_name]
-    fp = runs["fp64"]
+    fp = runs.get("fp64", head)
     energies = []
     for j, r in enumerate(head["recs"]):
         e = {"E": [r["E"].real, r["E"].imag], "nk": r["nk"], "seconds": r["seconds"],
              "dos": r["dos"], "mean_iter": float(r["iters"].mean())}
         for name in runs:
-            if name != "fp64":
+            if name != "fp64" and "fp64" in runs:
                 e[f"{name}_error_vs_fp64"] = O.error_report(runs[name]["recs"][j]["tau_bz"],
                                                             fp["recs"][j]["tau_bz"])
         energies.append(e)
@@ -126,7 +130,7 @@ def contour_bench(args, cfg, sy):
                "energies": energies}
     for name, r in runs.items():
         cfg_out[name] = {"k_points_per_s": nk_total / r["sec"], "seconds_per_contour": r["sec"]}
-        if name != "fp64":
+        if name != "fp64" and "fp64" in runs:
             cfg_out[name]["over_fp64"] = fp["sec"] / r["sec"]
             cfg_out[name]["integrated_trG_rel_err"] = abs(r["total"] - fp["total"]) / abs(fp["total"])
             cfg_out[name]["max_tau_rel_err"] = max(e[f"{name}_error_vs_fp64"]["tau_rel_err"]
