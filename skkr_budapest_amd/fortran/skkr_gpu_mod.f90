@@ -9,7 +9,7 @@ module skkr_gpu
        SKKR_ERR_NOMEM = 3, SKKR_ERR_SINGULAR = 4, SKKR_ERR_SGF_DIVERGED = 5, &
        SKKR_ERR_SGF_ITERMAX = 6, SKKR_ERR_UNSUPPORTED = 7, SKKR_ERR_STR_NOCONV = 8, &
        SKKR_ERR_TOO_LARGE = 9
-  integer(c_int), parameter :: SKKR_FP64 = 0, SKKR_MIXED_FP32 = 1, SKKR_MIXED_BF16 = 2
+  integer(c_int), parameter :: SKKR_FP64 = 0, SKKR_MIXED_FP32 = 1, SKKR_MIXED_BF16 = 2, SKKR_MIXED_INT8 = 3
 
   type, bind(C) :: skkr_config
      integer(c_int) :: lk, sites_per_pl, n_pl
