@@ -349,6 +349,8 @@ def main():
     ap.add_argument("--modes", default="",
                     help="comma list of k-loop modes to time (fp64, mixed_int8, mixed_bf16); "
                          "default all (fp64 alone with --fp64-only)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="k-points per launch group (0: the library's choice)")
     ap.add_argument("--kkr", action="store_true",
                     help="k-loop on the layer structure-constant producer (Nb(110) stack)")
     ap.add_argument("--kkr-sample", action="store_true",
@@ -391,7 +393,7 @@ def main():
     # each sees the same clock and temperature of the device
     ctxs = {}
     for name, prec in modes:
-        ctxs[name] = gpu.Context.for_system(sy, precision=prec)
+        ctxs[name] = gpu.Context.for_system(sy, precision=prec, batch=args.batch)
         ctxs[name].set_tinv(*sy.tinv(E))
     runs = kloop_steps(ctxs, E, k, w, args)
     for ctx in ctxs.values():

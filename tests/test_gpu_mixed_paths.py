@@ -1,0 +1,85 @@
+"""The mixed-precision modes (INT8 digit products, split-BF16 products) on the
+k-loop paths beside the Fourier producer: the layer structure constants
+(skkr_set_structure_kkr, lm and kappa-mu bases) and the off-diagonal site-pair
+sums (skkr_tau_bz_pairs), each against the FP64 oracle of the same inputs.
+
+Tolerance: relative error <= 1e-8 of the BZ-integrated tau (and of the pair
+sums), the accuracy the mixed modes are built for; the decimation iteration
+counts stay within one of FP64's."""
+import numpy as np
+import pytest
+
+import oracle
+from skkr_budapest_amd import systems as S
+from test_gpu_gstore import LMAX, _kkr_system
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-8
+MODES = ["MIXED_INT8", "MIXED_BF16"]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from skkr_budapest_amd import gpu as G
+    return G
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("relm", [0, 1])
+def test_mixed_kloop_on_layer_structure_constants(gpu, mode, relm):
+    """Nb(110) stack of the reference's sample, lmax 2 (relm 0: lm basis, 9
+    per site; relm 1: kappa-mu, 18 per site), 4 k-points, E = 0.5 + 0.05i."""
+    G = gpu
+    E = complex(0.5, 0.05)
+    g, al, ti = _kkr_system(E)
+    s, nl2 = g["s"], (LMAX + 1) ** 2
+    P = 3 if relm == 0 else 2
+    if relm:
+        ti = np.array([oracle.relmtrx(t, 1.0, LMAX) for t in ti])
+    tinv = np.concatenate([ti] * P)
+    ks = np.array([[0.2, -0.1], [-0.45, 0.3], [0.05, 0.6], [0.33, 0.12]])
+    ws = np.full(len(ks), 1.0 / len(ks))
+    L = oracle.Lattice(g["a1"], g["a2"])
+    ref, its = oracle.kkr_tau_bz(L, LMAX, s, g["nextra"], g["cvec"], al, E, g["eta"], tinv, ti, ti, ks, ws,
+                                 rel=relm)
+    lk = nl2 * (2 if relm else 1)
+    print(f"{mode} k-loop on the layer producer, rel={relm}", flush=True)
+    with G.Structure(LMAX, s, g["nextra"], g["eta"], g["a1"], g["a2"], g["cvec"]) as st:
+        with G.Context(lk=lk, sites_per_pl=s, n_pl=P, precision=getattr(G, mode)) as ctx:
+            ctx.set_tinv(tinv, ti, ti)
+            if relm:
+                ctx.set_structure_kkr(st, al, rel=relm)
+            else:
+                ctx.set_structure_kkr(st, al)
+            r = ctx.tau_bz(E, ks, ws)
+    assert (r["status"] == 0).all()
+    assert np.abs(r["iters"] - its).max() <= 1
+    e = rel(r["tau_bz"], ref)
+    print("  tau rel. error", e, r["stats"]["dom_name"], flush=True)
+    assert e < MIXED_TOL
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_pair_sums(gpu, mode):
+    """Off-diagonal tau of site pairs with phase-weighted BZ sums (the cluster
+    embedding input) in the mixed modes, against the dense-inverse oracle."""
+    sy = S.make_system(S.small_config(lk=16, s=2, P=4, r2max=8, tmodel="herm", energies=(0.35 + 0.04j,)))
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(5)
+    a1, a2 = (1.0, 0.0), (0.0, 1.0)
+    ns = 4 * 2
+    pairs = [(1, 1, 1, 0), (0, 1, 0, 0), (0, 5, 1, -1), (ns - 1, 0, 0, 2)]
+    ref_ij, ref_ji = oracle.tau_pairs(sy, E, k, w, pairs, a1, a2)
+    print(f"{mode} pair sums", flush=True)
+    with gpu.Context.for_system(sy, precision=getattr(gpu, mode)) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ctx.set_pairs(pairs, a1, a2)
+        r = ctx.tau_bz_pairs(E, k, w)
+    for q in range(len(pairs)):
+        assert rel(r["tau_ij"][q], ref_ij[q]) < MIXED_TOL, q
+        assert rel(r["tau_ji"][q], ref_ji[q]) < MIXED_TOL, q
