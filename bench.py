@@ -406,10 +406,11 @@ def main():
                "precision": head_name, "nk": nk}
     for name, r in runs.items():
         cfg_out[name] = mode_summary(r, nk, cfg)
-    if "mixed_int8" in runs and "fp64" in runs and "mixed_bf16" in runs:
-        from skkr_budapest_amd import observables as O
+    from skkr_budapest_amd import observables as O
+    if "mixed_int8" in runs and "fp64" in runs:
         cfg_out["mixed_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_int8"]["ms"]
         cfg_out["mixed_error_vs_fp64"] = O.error_report(runs["mixed_int8"]["tau_bz"], runs["fp64"]["tau_bz"])
+    if "mixed_bf16" in runs and "fp64" in runs:
         cfg_out["mixed_bf16_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_bf16"]["ms"]
         cfg_out["mixed_bf16_error_vs_fp64"] = O.error_report(runs["mixed_bf16"]["tau_bz"],
                                                              runs["fp64"]["tau_bz"])
@@ -500,7 +501,7 @@ def rooflines(run):
         per = flops / launches
         if cls == "dom" and stats[-1]["dom_name"].startswith("zgemm_i8"):
             exe = sum(s["dom_flops_exec"] for s in stats) / (ms * 1e-3) / 1e12
-            ent = {"bound": "mfma", "kernel": "zgemm_i8_kernel", "achieved": exe,
+            ent = {"bound": "mfma", "kernel": stats[-1]["dom_name"], "achieved": exe,
                    "peak": PEAK_INT8_TOPS, "unit": "TOP/s", "frac": exe / PEAK_INT8_TOPS,
                    "achieved_is": "executed INT8 MFMA ops",
                    "algorithmic_tflops_fp64_equivalent": alg,

@@ -190,7 +190,10 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
  *               that many slices (0: FP64)
  *   "zgemm_int8" 0 | 2..6: skkr_zgemm_batch runs the INT8 digit product with
- *               that many digits (0: FP64) */
+ *               that many digits (0: FP64)
+ *   "int8_tile" 64 | 128: work-group tile of the INT8 digit product (64 x 64,
+ *               two work-groups per CU, the default; or 128 x 64, one per
+ *               CU); the same bits (also SKKR_INT8_TILE in the environment) */
 int skkr_set_option(skkr_ctx *ctx, const char *key, int value);
 
 typedef struct {

@@ -138,6 +138,7 @@ struct skkr_ctx {
   // kind, and the choices per phase
   int split = 0;
   int split_i8 = 0;       // 1: INT8 digit products (zgemm_i8.h), 0: BF16 slices
+  int i8_tile = 64;       // INT8 kernel: 64 (64 x 64 tiles, two per CU) or 128 (128 x 64, one per CU)
   int split_dec = 5;      // decimation and lead self-energies, Im E >= fp64_dec_below_imE
   int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
   int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
@@ -449,28 +450,43 @@ static void gemm_tile(const skkr_ctx* c, int m, int n, int* tm, int* tiles) {
 // Split-BF16 product (zgemm_split.h) with S = c->split slices per operand,
 // or with c->split_i8 the INT8 digit product (zgemm_i8.h) with S digits.
 static int split_tiles(int m, int n) { return ((m + 63) / 64) * ((n + 63) / 64); }
-static int i8_tiles(int m, int n) { return ((m + 127) / 128) * ((n + 63) / 64); }
-static int split_tiles(const skkr_ctx* c, int m, int n) { return c->split_i8 ? i8_tiles(m, n) : split_tiles(m, n); }
+static int i8_tiles(const skkr_ctx* c, int m, int n) {
+  const int t = c->i8_tile;
+  return ((m + t - 1) / t) * ((n + 63) / 64);
+}
+static int split_tiles(const skkr_ctx* c, int m, int n) { return c->split_i8 ? i8_tiles(c, m, n) : split_tiles(m, n); }
 
 static int launch_zgemm_i8(skkr_ctx* c, int m, int n, int k, z** A, int lda, z** B, int ldb,
                            z** C, int ldc, z alpha, z beta, int nb, const int* active,
                            const EpiArgs& epi) {
-  dim3 grid(i8_tiles(m, n), nb);
+  dim3 grid(i8_tiles(c, m, n), nb);
   dom_begin(c);
   const z* const* a = (const z* const*)A;
   const z* const* b = (const z* const*)B;
-  switch (c->split) {
-    case 2: zgemm_i8_kernel<2><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
-    case 3: zgemm_i8_kernel<3><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
-    case 4: zgemm_i8_kernel<4><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
-    case 5: zgemm_i8_kernel<5><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
-    case 6: zgemm_i8_kernel<6><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
-    default: return SKKR_ERR_UNSUPPORTED;
+  if (c->i8_tile == 64) {
+    switch (c->split) {
+      case 2: zgemm_i8_64_kernel<2><<<grid, 256, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 3: zgemm_i8_64_kernel<3><<<grid, 256, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 4: zgemm_i8_64_kernel<4><<<grid, 256, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 5: zgemm_i8_64_kernel<5><<<grid, 256, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 6: zgemm_i8_64_kernel<6><<<grid, 256, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      default: return SKKR_ERR_UNSUPPORTED;
+    }
+  } else {
+    switch (c->split) {
+      case 2: zgemm_i8_kernel<2><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 3: zgemm_i8_kernel<3><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 4: zgemm_i8_kernel<4><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 5: zgemm_i8_kernel<5><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      case 6: zgemm_i8_kernel<6><<<grid, 512, 0, c->stream>>>(m, n, k, a, lda, b, ldb, C, ldc, alpha, beta, active, epi); break;
+      default: return SKKR_ERR_UNSUPPORTED;
+    }
   }
   HIPCHK(hipGetLastError());
   // executed INT8 MFMA work: S (S + 1) / 2 digit pairs x 4 real products on
-  // the padded 128 x 64 x 32 tiles
-  const double pm = (m + 127) / 128 * 128.0, pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
+  // the padded tiles (i8_tile x 64 x 32)
+  const int ti = c->i8_tile;
+  const double pm = (double)((m + ti - 1) / ti * ti), pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
   const double pairs = c->split * (c->split + 1) / 2.0;
   if (c->timing_detail && !c->capturing) c->dom_flops_exec += pairs * 4.0 * 2.0 * pm * pn * pk * nb;
   dom_end(c, 8.0 * m * n * (double)k * nb,
@@ -990,6 +1006,10 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
     const int t = atoi(e);
     if (t == 32 || t == 64 || t == 96 || t == 128) c->gemm_tile_force = t;
   }
+  if (const char* e = getenv("SKKR_INT8_TILE")) {
+    const int t = atoi(e);
+    if (t == 64 || t == 128) c->i8_tile = t;
+  }
   c->n = cfg->lk * cfg->sites_per_pl;
   c->nn = c->n * c->n;
   c->ns = cfg->sites_per_pl;
@@ -1396,7 +1416,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
       const double corr = (it_sum - rounds * B2d) * 3.0 * 16.0 * n3 + (started - B2d) * 2.0 * 8.0 * n3;
       c->dom_flops += corr;
       if (s_dec > 0) {  // executed split work of the same launches (padded tiles)
-        const int tm = c->split_i8 ? 128 : 64;
+        const int tm = c->split_i8 ? c->i8_tile : 64;
         const double pad = (double)((2 * c->n + tm - 1) / tm * tm) * ((c->n + 63) / 64 * 64) *
                            ((c->n + 31) / 32 * 32) / (2.0 * n3);
         c->dom_flops_exec += corr * s_dec * (s_dec + 1) / 2.0 * pad;
@@ -1477,7 +1497,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     st->inv_bytes = c->inv_bytes;
     st->redo_batches = c->redo_batches;
     snprintf(st->dom_name, sizeof(st->dom_name), "%s",
-             (s_dec > 0 && s_sw > 0) ? (c->split_i8 ? "zgemm_i8_kernel" : "zgemm_split_kernel")
+             (s_dec > 0 && s_sw > 0) ? (c->split_i8 ? (c->i8_tile == 64 ? "zgemm_i8_64_kernel" : "zgemm_i8_kernel") : "zgemm_split_kernel")
                                      : (s_sw > 0 ? "zgemm_f64+split" : "zgemm_f64_kernel"));
   }
   c->timing_detail = 0;
@@ -1560,6 +1580,12 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     if (value != 0 && (value < 2 || value > (i8 ? 6 : 5))) return SKKR_ERR_ARG;
     c->zgemm_split = value;
     c->zgemm_i8 = i8 ? 1 : 0;
+    return SKKR_OK;
+  }
+  if (strcmp(key, "int8_tile") == 0) {
+    if (value != 64 && value != 128) return SKKR_ERR_ARG;
+    c->i8_tile = value;
+    drop_graphs(c);
     return SKKR_OK;
   }
   if (strcmp(key, "gemm_3m") == 0) {

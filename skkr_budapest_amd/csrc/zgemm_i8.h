@@ -27,8 +27,10 @@
 // and a CPU emulation of the same steps reproduces it bit for bit.  The error
 // per product is about 2^-(8S-3) K max|A_row| max|B_col| per chunk.
 //
-// Work-group tile 128 x 64 of C, eight waves of 32 x 32 (one MFMA block
-// each); one work-group per CU (LDS).  All 512 threads cut the next chunk into LDS digit
+// zgemm_i8_kernel: work-group tile 128 x 64 of C, eight waves of 32 x 32 (one
+// MFMA block each); one work-group per CU (LDS).  (zgemm_i8_64_kernel at the
+// end: 64 x 64 tiles, two work-groups per CU, the same bits.)  All 512 threads
+// cut the next chunk into LDS digit
 // planes [digit][re|im][row or column][32 bytes of K] between the current
 // chunk's digit pairs; the planes are double-buffered (S = 6: 2 x 72 KB), one
 // barrier per chunk.  Each wave reads the A and B digits of its 32 rows and
@@ -158,6 +160,176 @@ __device__ __forceinline__ void i8_fetch(const double2* __restrict__ M, long lon
   }
 }
 
+// The digit products of one K-chunk for a wave's 32 x 32 block of C and
+// their fold into (cre, cim): planes pa (A, TM rows) and pb (B, TN columns),
+// exponents ea_ / eb_, the lane's plane offsets oA / oB, its C row rA (in the
+// tile), wave column wn and K-half hf.  after_h(h) runs at the end of group
+// pair h, before its fold (the 128 x 64 kernel cuts the next chunk there).
+template <int S, int TM, int TN, class F>
+__device__ __forceinline__ void i8_chunk_products(const unsigned* __restrict__ pa, const unsigned* __restrict__ pb,
+                                                  const int* __restrict__ ea_, const int* __restrict__ eb_,
+                                                  int oA, int oB, int rA, int wn, int hf, double* cre,
+                                                  double* cim, F&& after_h) {
+  // the digit pairs of the chunk in order: group pair h, group g, A digit
+  // S-1-i with B digit S-1-(g-i); the operands of pair p + 1 are read before
+  // the products of pair p are issued
+  constexpr int NP = S * (S + 1) / 2;
+  // group of pair p: pairs g (g + 1) / 2 .. (g + 1) (g + 2) / 2 - 1 (no loop,
+  // so that the pair loop unrolls completely)
+  auto pair_g = [](int p) { return (p >= 1) + (p >= 3) + (p >= 6) + (p >= 10) + (p >= 15); };
+  auto pair_i = [&](int p) { return p - pair_g(p) * (pair_g(p) + 1) / 2; };
+  i8x4w op[2][4];
+  auto load = [&](int p, i8x4w* o) {
+    const int g = pair_g(p), i = pair_i(p);
+    const int ta = S - 1 - i, tb = S - 1 - (g - i);
+    o[0] = *reinterpret_cast<const i8x4w*>(pa + (ta * 2 + 0) * TM * 8 + oA);
+    o[1] = *reinterpret_cast<const i8x4w*>(pa + (ta * 2 + 1) * TM * 8 + oA);
+    o[2] = *reinterpret_cast<const i8x4w*>(pb + (tb * 2 + 0) * TN * 8 + oB);
+    o[3] = *reinterpret_cast<const i8x4w*>(pb + (tb * 2 + 1) * TN * 8 + oB);
+  };
+  i32x16 ap, am, ai;
+  load(0, op[0]);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int g = pair_g(p), i = pair_i(p), h = g / 2;
+    if (p + 1 < NP) load(p + 1, op[(p + 1) & 1]);
+    const i8x4w* o = op[p & 1];  // A re, A im, B re, B im
+    if (g == 2 * h + 1 && i == 0) {
+      // group 2h is complete: P = 256 G_2h, and group 2h + 1 adds to it
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        ap[r] <<= 8;
+        am[r] <<= 8;
+        ai[r] <<= 8;
+      }
+    }
+    if (g == 2 * h && i == 0) {
+      const i32x16 z = {0};
+      ap = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[0], z, 0, 0, 0);
+      am = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[1], z, 0, 0, 0);
+      ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[1], z, 0, 0, 0);
+    } else {
+      ap = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[0], ap, 0, 0, 0);
+      am = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[1], am, 0, 0, 0);
+      ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[1], ai, 0, 0, 0);
+    }
+    ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[0], ai, 0, 0, 0);
+    // two pairs' operands in flight at a time
+    __builtin_amdgcn_sched_barrier(0);
+    const bool last_of_h = (p + 1 == NP) || (pair_g(p + 1) / 2 != h);
+    if (last_of_h) {
+      if (g == 2 * h) {  // odd S: the last group pair has one group
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          ap[r] <<= 8;
+          am[r] <<= 8;
+          ai[r] <<= 8;
+        }
+      }
+      after_h(h);
+      // fold: C += P_h 2^(eA + eB - 20 - 16 h)
+      const int ea = ea_[rA] - 20 - 16 * h;
+#pragma unroll
+      for (int a4 = 0; a4 < 4; ++a4) {
+        const i8x4w eb = *reinterpret_cast<const i8x4w*>(eb_ + wn * 32 + 8 * a4 + 4 * hf);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 4 * a4 + q;
+          cre[r] += ldexp((double)(ap[r] - am[r]), ea + eb[q]);
+          cim[r] += ldexp((double)ai[r], ea + eb[q]);
+        }
+      }
+      // the fold happens here (not sunk past the next pairs), so that only
+      // one set of accumulators is live
+#pragma unroll
+      for (int r = 0; r < 16; ++r) asm volatile("" : "+v"(cre[r]), "+v"(cim[r]));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
+// Write the wave's 32 x 32 block (lane: C row row0 + rA; register r: column
+// col0 + 32 wn + 8 (r / 4) + 4 hf + r % 4) with the plain or the decimation
+// epilogue.  red: NT doubles of free LDS for the work-group's |alpha'|^2 sum.
+template <int NT>
+__device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, int rA, int wn, int hf,
+                                            const double* cre, const double* cim, double2* __restrict__ C,
+                                            int ldc, double2 alpha, double2 beta, const EpiArgs& epi, int b,
+                                            int tile, double* red, int tid) {
+  const int gr = row0 + rA;
+  auto col_of = [&](int r) { return col0 + wn * 32 + 8 * (r >> 2) + 4 * hf + (r & 3); };
+  if (epi.mode == EPI_PLAIN) {
+    const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+    if (gr < m) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gc = col_of(r);
+        if (gc < n) {
+          double2 v = cmul(alpha, make_double2(cre[r], cim[r]));
+          const long long o = gr + (long long)gc * ldc;
+          if (use_beta) v = cadd(v, cmul(beta, C[o]));
+          C[o] = v;
+        }
+      }
+    }
+    return;
+  }
+  const int hn = epi.n;
+  double2* __restrict__ eps = epi.eps[b];
+  double2* __restrict__ xx = epi.x[b];
+  double2* __restrict__ abn = epi.abn[b];
+  double part = 0.0;
+  if (gr < m) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gc = col_of(r);
+      if (gc < n) {
+        const double2 v = make_double2(cre[r], cim[r]);
+        if (gr < hn) {
+          const long long o = gr + (long long)gc * hn;
+          if (epi.mode == EPI_DEC_BETA) {
+            eps[o] = csub(eps[o], v);
+            xx[o] = csub(xx[o], v);
+          } else {
+            abn[gr + (long long)gc * 2 * hn] = v;
+            part += v.x * v.x + v.y * v.y;
+          }
+        } else {
+          const long long o = (gr - hn) + (long long)gc * hn;
+          if (epi.mode == EPI_DEC_BETA)
+            abn[gr + (long long)gc * 2 * hn] = v;
+          else
+            eps[o] = csub(eps[o], v);
+        }
+      }
+    }
+  }
+  if (epi.mode == EPI_DEC_ALPHA) {
+    // deterministic work-group sum of |alpha'|^2 (fixed-order tree in LDS)
+    red[tid] = part;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (tid < st) red[tid] += red[tid + st];
+      __syncthreads();
+    }
+    if (tid == 0) epi.norm[(long long)b * epi.tiles + tile] = red[0];
+  }
+}
+
+// XCD-aware numbering as in zgemm_f64_kernel: the tiles of one product on
+// one XCD (work-groups go round-robin over the eight XCDs)
+__device__ __forceinline__ void i8_tile_of(int& b, int& tile) {
+  b = blockIdx.y;
+  tile = blockIdx.x;
+  const int tiles = gridDim.x;
+  const int L = blockIdx.x + tiles * blockIdx.y;
+  if (tiles > 1 && L < 8 * tiles * (int)(gridDim.y / 8)) {
+    const int g = L / (8 * tiles), rem = L % (8 * tiles);
+    tile = rem / 8;
+    b = 8 * g + rem % 8;
+  }
+}
+
 template <int S>
 __global__ __launch_bounds__(512) void zgemm_i8_kernel(
     int m, int n, int k, const double2* const* __restrict__ Ap, int lda,
@@ -172,18 +344,8 @@ __global__ __launch_bounds__(512) void zgemm_i8_kernel(
   __shared__ __attribute__((aligned(16))) int eA[2 * TM];
   __shared__ __attribute__((aligned(16))) int eB[2 * TN];
 
-  // XCD-aware numbering as in zgemm_f64_kernel: the tiles of one product on
-  // one XCD
-  int b = blockIdx.y, tile = blockIdx.x;
-  {
-    const int tiles = gridDim.x;
-    const int L = blockIdx.x + tiles * blockIdx.y;
-    if (tiles > 1 && L < 8 * tiles * (int)(gridDim.y / 8)) {
-      const int g = L / (8 * tiles), rem = L % (8 * tiles);
-      tile = rem / 8;
-      b = 8 * g + rem % 8;
-    }
-  }
+  int b, tile;
+  i8_tile_of(b, tile);
   if (active && !active[b]) return;
   const int tiles_m = (m + TM - 1) / TM;
   const int row0 = (tile % tiles_m) * TM;
@@ -238,164 +400,103 @@ __global__ __launch_bounds__(512) void zgemm_i8_kernel(
   for (int ch = 0; ch < nchunk; ++ch) {
     const int buf = ch & 1;
     const bool more = ch + 1 < nchunk;
-    const unsigned* __restrict__ pa = sA + buf * PA;
-    const unsigned* __restrict__ pb = sB + buf * PB;
-    const int* __restrict__ ea_ = eA + buf * TM;
-    const int* __restrict__ eb_ = eB + buf * TN;
     if (more) fetch((ch + 1) * KC);
-    // the digit pairs of the chunk in order: group pair h, group g, A digit
-    // S-1-i with B digit S-1-(g-i); the operands of pair p + 1 are read
-    // before the products of pair p are issued
-    constexpr int NP = S * (S + 1) / 2;
-    // group of pair p: pairs g (g + 1) / 2 .. (g + 1) (g + 2) / 2 - 1 (no loop,
-    // so that the pair loop unrolls completely)
-    auto pair_g = [](int p) { return (p >= 1) + (p >= 3) + (p >= 6) + (p >= 10) + (p >= 15); };
-    auto pair_i = [&](int p) { return p - pair_g(p) * (pair_g(p) + 1) / 2; };
-    i8x4w op[2][4];
-    auto load = [&](int p, i8x4w* o) {
-      const int g = pair_g(p), i = pair_i(p);
-      const int ta = S - 1 - i, tb = S - 1 - (g - i);
-      o[0] = *reinterpret_cast<const i8x4w*>(pa + (ta * 2 + 0) * TM * 8 + oA);
-      o[1] = *reinterpret_cast<const i8x4w*>(pa + (ta * 2 + 1) * TM * 8 + oA);
-      o[2] = *reinterpret_cast<const i8x4w*>(pb + (tb * 2 + 0) * TN * 8 + oB);
-      o[3] = *reinterpret_cast<const i8x4w*>(pb + (tb * 2 + 1) * TN * 8 + oB);
+    // the next chunk's split work between the group pairs (its loads have
+    // had the first pair's products to arrive)
+    auto after_h = [&](int h) {
+      if (!more) return;
+      if (H >= 3) {
+        if (h == 1) store(0, buf ^ 1);
+        if (h == 2) {
+          store(1, buf ^ 1);
+          store(2, buf ^ 1);
+        }
+      } else if (h == H - 1) {
+        store(0, buf ^ 1);
+        store(1, buf ^ 1);
+        store(2, buf ^ 1);
+      }
     };
-    i32x16 ap, am, ai;
-    load(0, op[0]);
-#pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const int g = pair_g(p), i = pair_i(p), h = g / 2;
-      if (p + 1 < NP) load(p + 1, op[(p + 1) & 1]);
-      const i8x4w* o = op[p & 1];  // A re, A im, B re, B im
-      if (g == 2 * h + 1 && i == 0) {
-        // group 2h is complete: P = 256 G_2h, and group 2h + 1 adds to it
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          ap[r] <<= 8;
-          am[r] <<= 8;
-          ai[r] <<= 8;
-        }
-      }
-      if (g == 2 * h && i == 0) {
-        const i32x16 z = {0};
-        ap = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[0], z, 0, 0, 0);
-        am = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[1], z, 0, 0, 0);
-        ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[1], z, 0, 0, 0);
-      } else {
-        ap = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[0], ap, 0, 0, 0);
-        am = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[1], am, 0, 0, 0);
-        ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[2], o[1], ai, 0, 0, 0);
-      }
-      ai = __builtin_amdgcn_mfma_i32_32x32x32_i8(o[3], o[0], ai, 0, 0, 0);
-      // two pairs' operands in flight at a time
-      __builtin_amdgcn_sched_barrier(0);
-      const bool last_of_h = (p + 1 == NP) || (pair_g(p + 1) / 2 != h);
-      if (last_of_h) {
-        if (g == 2 * h) {  // odd S: the last group pair has one group
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            ap[r] <<= 8;
-            am[r] <<= 8;
-            ai[r] <<= 8;
-          }
-        }
-        // the next chunk's split work between the group pairs (its loads
-        // have had the first pair's products to arrive)
-        if (more) {
-          if (H >= 3) {
-            if (h == 1) store(0, buf ^ 1);
-            if (h == 2) {
-              store(1, buf ^ 1);
-              store(2, buf ^ 1);
-            }
-          } else if (h == H - 1) {
-            store(0, buf ^ 1);
-            store(1, buf ^ 1);
-            store(2, buf ^ 1);
-          }
-        }
-        // fold: C += P_h 2^(eA + eB - 20 - 16 h)
-        const int ea = ea_[rA] - 20 - 16 * h;
-#pragma unroll
-        for (int a4 = 0; a4 < 4; ++a4) {
-          const i8x4w eb = *reinterpret_cast<const i8x4w*>(eb_ + wn * 32 + 8 * a4 + 4 * hf);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = 4 * a4 + q;
-            cre[r] += ldexp((double)(ap[r] - am[r]), ea + eb[q]);
-            cim[r] += ldexp((double)ai[r], ea + eb[q]);
-          }
-        }
-        // the fold happens here (not sunk past the next pairs), so that only
-        // one set of accumulators is live
-#pragma unroll
-        for (int r = 0; r < 16; ++r) asm volatile("" : "+v"(cre[r]), "+v"(cim[r]));
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
+    i8_chunk_products<S, TM, TN>(sA + buf * PA, sB + buf * PB, eA + buf * TM, eB + buf * TN, oA, oB, rA, wn,
+                                 hf, cre, cim, after_h);
     if (more) __syncthreads();
   }
-  __syncthreads();  // planes free for the reduction below
+  __syncthreads();  // planes free for the reduction in the epilogue
+  i8_epilogue<512>(m, n, row0, col0, rA, wn, hf, cre, cim, C, ldc, alpha, beta, epi, b, tile,
+                   reinterpret_cast<double*>(sA), tid);
+}
 
-  // lane: C row rA; register r: C column 8 (r / 4) + 4 hf + r % 4 of the wave's 32
-  const int gr = row0 + rA;
-  auto col_of = [&](int r) { return col0 + wn * 32 + 8 * (r >> 2) + 4 * hf + (r & 3); };
-  if (epi.mode == EPI_PLAIN) {
-    const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
-    if (gr < m) {
+// The same product on 64 x 64 tiles: four waves of 32 x 32, one set of digit
+// planes (S = 6: 48 KB), so that two work-groups share a CU and one's cut,
+// loads and epilogue run beside the other's MFMAs.  Per chunk: cut the
+// chunk's values into the planes, start the loads of the next chunk, barrier,
+// the digit products and folds, barrier.  The digits, the pair order and the
+// folds are those of zgemm_i8_kernel, so both give the same bits.
+template <int S>
+__global__ __launch_bounds__(256, 2) void zgemm_i8_64_kernel(
+    int m, int n, int k, const double2* const* __restrict__ Ap, int lda,
+    const double2* const* __restrict__ Bp, int ldb, double2* const* __restrict__ Cp, int ldc,
+    double2 alpha, double2 beta, const int* __restrict__ active, EpiArgs epi) {
+  static_assert(S >= 2 && S <= 6, "the FMA digit cut holds c + V below 2^51: S <= 6 (and pair_g covers 21 pairs)");
+  constexpr int TM = 64, TN = 64, KC = 32;
+  constexpr int PA = S * 2 * TM * 8;
+  constexpr int PB = S * 2 * TN * 8;
+  __shared__ __attribute__((aligned(16))) unsigned sA[PA];
+  __shared__ __attribute__((aligned(16))) unsigned sB[PB];
+  __shared__ __attribute__((aligned(16))) int eA[TM];
+  __shared__ __attribute__((aligned(16))) int eB[TN];
+
+  int b, tile;
+  i8_tile_of(b, tile);
+  if (active && !active[b]) return;
+  const int tiles_m = (m + TM - 1) / TM;
+  const int row0 = (tile % tiles_m) * TM;
+  const int col0 = (tile / tiles_m) * TN;
+  const double2* __restrict__ A = Ap[b];
+  const double2* __restrict__ B = Bp[b];
+  double2* __restrict__ C = Cp ? Cp[b] : nullptr;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w & 1, wn = w >> 1;  // wave: rows 32 wm .. +31, columns 32 wn .. +31
+  const int l32 = lane & 31, hf = lane >> 5;
+  // split items: lines tid / 8 and tid / 8 + 32 of A and of B; quarter tid % 8
+  const int irc = tid >> 3, iq = tid & 7;
+
+  double cre[16], cim[16];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int gc = col_of(r);
-        if (gc < n) {
-          double2 v = cmul(alpha, make_double2(cre[r], cim[r]));
-          const long long o = gr + (long long)gc * ldc;
-          if (use_beta) v = cadd(v, cmul(beta, C[o]));
-          C[o] = v;
-        }
-      }
+  for (int r = 0; r < 16; ++r) cre[r] = cim[r] = 0.0;
+
+  const bool full = (row0 + TM <= m) && (col0 + TN <= n) && (k % KC == 0);
+  const double2* Ar = A + row0;
+  const double2* Bc = B + (long long)col0 * ldb;
+  double2 v[4][4];
+  auto fetch = [&](int k0) {
+    if (full) {
+      i8_fetch<true>(Ar, 1, lda, m - row0, k0, k, irc, iq, v[0]);
+      i8_fetch<true>(Ar, 1, lda, m - row0, k0, k, irc + 32, iq, v[1]);
+      i8_fetch<true>(Bc, ldb, 1, n - col0, k0, k, irc, iq, v[2]);
+      i8_fetch<true>(Bc, ldb, 1, n - col0, k0, k, irc + 32, iq, v[3]);
+    } else {
+      i8_fetch<false>(Ar, 1, lda, m - row0, k0, k, irc, iq, v[0]);
+      i8_fetch<false>(Ar, 1, lda, m - row0, k0, k, irc + 32, iq, v[1]);
+      i8_fetch<false>(Bc, ldb, 1, n - col0, k0, k, irc, iq, v[2]);
+      i8_fetch<false>(Bc, ldb, 1, n - col0, k0, k, irc + 32, iq, v[3]);
     }
-    return;
-  }
-  const int hn = epi.n;
-  double2* __restrict__ eps = epi.eps[b];
-  double2* __restrict__ xx = epi.x[b];
-  double2* __restrict__ abn = epi.abn[b];
-  double part = 0.0;
-  if (gr < m) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int gc = col_of(r);
-      if (gc < n) {
-        const double2 v = make_double2(cre[r], cim[r]);
-        if (gr < hn) {
-          const long long o = gr + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA) {
-            eps[o] = csub(eps[o], v);
-            xx[o] = csub(xx[o], v);
-          } else {
-            abn[gr + (long long)gc * 2 * hn] = v;
-            part += v.x * v.x + v.y * v.y;
-          }
-        } else {
-          const long long o = (gr - hn) + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA)
-            abn[gr + (long long)gc * 2 * hn] = v;
-          else
-            eps[o] = csub(eps[o], v);
-        }
-      }
-    }
-  }
-  if (epi.mode == EPI_DEC_ALPHA) {
-    // deterministic work-group sum of |alpha'|^2 (fixed-order tree in LDS,
-    // in the digit planes, which are free now)
-    double* red = reinterpret_cast<double*>(sA);
-    red[tid] = part;
+  };
+  const int cB = wn * 32 + l32, rA = wm * 32 + l32;
+  const int oB = cB * 8 + 4 * i8_half(cB, hf);
+  const int oA = rA * 8 + 4 * i8_half(rA, hf);
+  const int nchunk = (k + KC - 1) / KC;
+  fetch(0);
+  for (int ch = 0; ch < nchunk; ++ch) {
+    i8_split_store<S>(v[0], sA, TM, eA, irc, iq);
+    i8_split_store<S>(v[1], sA, TM, eA, irc + 32, iq);
+    i8_split_store<S>(v[2], sB, TN, eB, irc, iq);
+    i8_split_store<S>(v[3], sB, TN, eB, irc + 32, iq);
+    if (ch + 1 < nchunk) fetch((ch + 1) * KC);
     __syncthreads();
-    for (int st = 256; st > 0; st >>= 1) {
-      if (tid < st) red[tid] += red[tid + st];
-      __syncthreads();
-    }
-    if (tid == 0) epi.norm[(long long)b * epi.tiles + tile] = red[0];
+    i8_chunk_products<S, TM, TN>(sA, sB, eA, eB, oA, oB, rA, wn, hf, cre, cim, [](int) {});
+    __syncthreads();  // planes read; free for the next cut (or the reduction)
   }
+  i8_epilogue<256>(m, n, row0, col0, rA, wn, hf, cre, cim, C, ldc, alpha, beta, epi, b, tile,
+                   reinterpret_cast<double*>(sA), tid);
 }

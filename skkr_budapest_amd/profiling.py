@@ -55,15 +55,20 @@ def run_split(n=128, nb=1024, slices=(0, 2, 3, 4, 5)):
             print(f"split: {nb} x complex {n}^3 with {s} slices per operand (0: FP64 kernel)", flush=True)
 
 
-def run_int8(n=128, nb=1024, digits=(0, 4, 5, 6)):
+def run_int8(n=128, nb=1024, digits=(0, 4, 5, 6), tiles=(128, 64)):
     from . import gpu
     a, b = _mats(n, nb, 1), _mats(n, nb, 2)
     with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
-        for s in digits:
-            ctx.set_option("zgemm_int8", s)
-            for _ in range(3):
-                ctx.zgemm(a, b)
-            print(f"int8: {nb} x complex {n}^3 with {s} digits per operand (0: FP64 kernel)", flush=True)
+        for t in tiles:
+            ctx.set_option("int8_tile", t)
+            for s in digits:
+                if s == 0 and t != tiles[0]:
+                    continue
+                ctx.set_option("zgemm_int8", s)
+                for _ in range(3):
+                    ctx.zgemm(a, b)
+                print(f"int8: {nb} x complex {n}^3 with {s} digits per operand (0: FP64 kernel), "
+                      f"INT8 tile {t}", flush=True)
 
 
 def run_inverse(n=128, nb=1024):

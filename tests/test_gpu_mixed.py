@@ -49,19 +49,24 @@ def test_split_gemm_matches_emulation_bitwise(gpu, n):
     assert np.linalg.norm(c64 - a @ b) < 1e-14 * np.linalg.norm(a @ b)
 
 
+@pytest.mark.parametrize("tile", [128, 64])
 @pytest.mark.parametrize("n", [36, 128, 200])
-def test_i8_gemm_matches_emulation_bitwise(gpu, n):
+def test_i8_gemm_matches_emulation_bitwise(gpu, n, tile):
     """The INT8 digit product for every digit count on ragged shapes (n = 36,
-    200: partial 128 x 64 tiles and K-chunks) and a batch of 11; rows and
-    columns scaled over 12 orders of magnitude."""
+    200: partial tiles and K-chunks) and a batch of 11; rows and columns
+    scaled over 12 orders of magnitude; both work-group tiles (128 x 64 and
+    64 x 64) give the emulation's bits."""
     rng = np.random.default_rng(2000 + n)
     nb = 11
     a = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
     b = rng.normal(size=(nb, n, n)) + 1j * rng.normal(size=(nb, n, n))
     a *= np.logspace(-6, 6, n)[None, :, None]
     b *= np.logspace(4, -8, n)[None, None, :]
-    print(f"INT8 GEMM n={n} nb={nb}", flush=True)
+    print(f"INT8 GEMM n={n} nb={nb} tile={tile}", flush=True)
     with gpu.Context(lk=n, sites_per_pl=1, n_pl=1) as ctx:
+        ctx.set_option("int8_tile", tile)
+        with pytest.raises(gpu.SkkrError):
+            ctx.set_option("int8_tile", 32)
         for s in (2, 3, 4, 5, 6):
             ctx.set_option("zgemm_int8", s)
             c = ctx.zgemm(a, b)
