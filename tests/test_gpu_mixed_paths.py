@@ -83,3 +83,29 @@ def test_mixed_pair_sums(gpu, mode):
     for q in range(len(pairs)):
         assert rel(r["tau_ij"][q], ref_ij[q]) < MIXED_TOL, q
         assert rel(r["tau_ji"][q], ref_ji[q]) < MIXED_TOL, q
+
+
+def test_int8_tiles_give_the_same_kloop(gpu):
+    """The INT8 k-loop on both work-group tiles (64 x 64, two per CU, and
+    128 x 64, one per CU) of config 1 on 16 k-points: the same digit products
+    and folds, so tau_BZ, the per-k traces and the decimation iteration counts
+    are bitwise equal (the |alpha'|^2 partial sums of the stop rule are summed
+    over different tile partitions; here they decide alike)."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.linspace(0, len(w) - 1, 16).astype(int)
+    ks, ws = k[idx], np.full(len(idx), 1.0 / len(idx))
+    out = {}
+    for tile in (64, 128):
+        print(f"INT8 k-loop, tile {tile}", flush=True)
+        with gpu.Context.for_system(sy, precision=gpu.MIXED_INT8) as ctx:
+            ctx.set_option("int8_tile", tile)
+            ctx.set_tinv(*sy.tinv(E))
+            out[tile] = ctx.tau_bz(E, ks, ws)
+    a, b = out[64], out[128]
+    assert (a["status"] == 0).all() and (b["status"] == 0).all()
+    np.testing.assert_array_equal(a["iters"], b["iters"])
+    np.testing.assert_array_equal(a["tau_bz"], b["tau_bz"])
+    np.testing.assert_array_equal(a["trk"], b["trk"])
