@@ -498,6 +498,29 @@ def test_physical_tau_adds_up_over_calls(gpu):
                                   whole["taukdiag"])
 
 
+def test_empty_and_single_k_calls(gpu):
+    """tau_bz accumulates (+=): a call with no k-points returns OK and leaves
+    the sum as it was; one k-point alone matches the oracle; the same k twice
+    in one call gives twice the sum of one (the rank-partial-sum contract
+    of the reference's k-loop, cpacoord.f:147-149,326)."""
+    sy = small_sys(lk=8, s=2, P=3, r2max=4, E=0.35 + 0.04j)
+    E = sy.cfg.energies[0]
+    k1, w1 = np.array([[0.21, -0.37]]), np.array([1.0])
+    ref = oracle.tau_bz(sy, E, k1, w1)
+    with gpu.Context.for_system(sy) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        one = ctx.tau_bz(E, k1, w1)
+        assert one["rc"] == 0 and (one["status"] == 0).all()
+        assert rel(one["tau_bz"], ref["tau_bz"]) < TOL
+        acc = one["tau_bz"].copy()
+        empty = ctx.tau_bz(E, np.zeros((0, 2)), np.zeros(0), tau_bz=acc)
+        assert empty["rc"] == 0 and empty["trk"].shape[0] == 0
+        np.testing.assert_array_equal(acc, one["tau_bz"])
+        two = ctx.tau_bz(E, np.concatenate([k1, k1]), np.concatenate([w1, w1]))
+    assert rel(two["tau_bz"], 2 * one["tau_bz"]) < 1e-14
+    np.testing.assert_array_equal(two["iters"][0], two["iters"][1])
+
+
 def test_set_option_rejects_unknown(gpu):
     with gpu.Context(lk=4, sites_per_pl=1, n_pl=1) as ctx:
         with pytest.raises(gpu.SkkrError):
