@@ -257,3 +257,62 @@ def test_phystau_closed_forms():
     lhs = oracle.phystau((w[:, None, None] * taus).sum(0), t, t, av)
     rhs = sum(wi * oracle.phystau(ti, t, t, av) for wi, ti in zip(w, taus))
     assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(lhs).max()
+
+
+def _lapack_inverse(a):
+    """The inverse as dbogo's gjinv2.f forms it: LAPACK zgetrf, then zgetri
+    (skkr_dbogo_bsf/main/gjinv2.f:21,48,55), here through scipy's LAPACK."""
+    from scipy.linalg import lapack
+    lu, piv, info = lapack.zgetrf(a)
+    assert info == 0
+    inv, info = lapack.zgetri(lu, piv)
+    assert info == 0
+    return inv
+
+
+def _surfgfd_with(inverse, ml, m0, mr, itermax=50, errmax=1e-8):
+    """surfgf.f:38-105 restated with a pluggable inverse: eps = x = M0,
+    alpha = -ML, beta = -MR; per iteration y = eps^-1, alpha <- alpha y alpha,
+    beta <- beta y beta, eps <- eps - alpha y beta - beta y alpha,
+    x <- x - alpha y beta, stop when ||alpha||_F < errmax; g_s = x^-1."""
+    eps, x = m0.copy(), m0.copy()
+    al, be = -ml, -mr
+    for it in range(1, itermax + 1):
+        y = inverse(eps)
+        ay, by = al @ y, be @ y
+        eps = eps - ay @ be - by @ al
+        x = x - ay @ be
+        al, be = ay @ al, by @ be
+        if np.linalg.norm(al) < errmax:
+            return inverse(x), it
+    raise AssertionError("no convergence")
+
+
+def test_dbogo_lapack_inverse_path_matches_gauss_jordan():
+    """dbogo inverts with LAPACK (gjinv2.f); the golden vectors were made with
+    the Gauss-Jordan gjinv.f linked in its place.  On the bdg18 system (e/h
+    blocks, E = 0.35 + 0.06i) the LAPACK inverse of t^-1 - G(k) and the
+    decimation run on LAPACK inverses give the oracle's (Gauss-Jordan) results
+    to round-off, with the same iteration counts."""
+    sy = GOLDEN["bdg18"]
+    g = np.load(os.path.join(GOLD, "bdg18.npz"))
+    E = complex(g["E"])
+    ti, tl, _ = sy.tinv(E)
+    lk, s = sy.lk, sy.s
+    for k in g["k"][::9]:
+        m00, m01, m10 = oracle.pl_blocks(oracle.fourier(sy.R, sy.G_intra, k),
+                                         oracle.fourier(sy.R, sy.G_inter, k))
+        a = m00.copy()
+        for i in range(s):
+            a[i * lk:(i + 1) * lk, i * lk:(i + 1) * lk] += ti[i]
+        gj, st = oracle.gjinv(a)
+        assert st == 0
+        assert rel(_lapack_inverse(a), gj) < 1e-13
+        m0 = m00.copy()
+        for i in range(s):
+            m0[i * lk:(i + 1) * lk, i * lk:(i + 1) * lk] += tl[i]
+        ref, it_ref, st = oracle.surfgfd(m10, m0, m01)
+        assert st == 0
+        gs, it = _surfgfd_with(_lapack_inverse, m10, m0, m01)
+        assert it == it_ref
+        assert rel(gs, ref) < 1e-12
