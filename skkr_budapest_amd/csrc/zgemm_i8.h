@@ -248,6 +248,19 @@ __device__ __forceinline__ void i8_chunk_products(const unsigned* __restrict__ p
   }
 }
 
+// Global (not flat) loads and stores of a complex value: they count on the
+// vector-memory counter only, so a store waits for nothing that LDS does
+typedef double __attribute__((address_space(1))) i8_gdbl;
+__device__ __forceinline__ double2 i8_gload(const double2* p) {
+  const i8_gdbl* q = (const i8_gdbl*)(const double*)p;
+  return make_double2(q[0], q[1]);
+}
+__device__ __forceinline__ void i8_gstore(double2* p, double2 v) {
+  i8_gdbl* q = (i8_gdbl*)(double*)p;
+  q[0] = v.x;
+  q[1] = v.y;
+}
+
 // Write the wave's 32 x 32 block (lane: C row row0 + rA; register r: column
 // col0 + 32 wn + 8 (r / 4) + 4 hf + r % 4) with the plain or the decimation
 // epilogue.  red: NT doubles of free LDS for the work-group's |alpha'|^2 sum.
@@ -267,8 +280,8 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
         if (gc < n) {
           double2 v = cmul(alpha, make_double2(cre[r], cim[r]));
           const long long o = gr + (long long)gc * ldc;
-          if (use_beta) v = cadd(v, cmul(beta, C[o]));
-          C[o] = v;
+          if (use_beta) v = cadd(v, cmul(beta, i8_gload(C + o)));
+          i8_gstore(C + o, v);
         }
       }
     }
@@ -280,26 +293,41 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
   double2* __restrict__ abn = epi.abn[b];
   double part = 0.0;
   if (gr < m) {
+    // Top rows (gr < hn): beta mode eps -= v and x -= v, alpha mode alpha' =
+    // v; bottom rows: beta mode beta' = v, alpha mode eps -= v.  A lane's 16
+    // values share its row, so each read-modify-write issues all its loads
+    // before the first store (a column beyond n reads column n - 1 and stores
+    // nothing): their latencies overlap instead of adding up.
+    const bool top = gr < hn;
+    const bool rmw_eps = top ? epi.mode == EPI_DEC_BETA : epi.mode == EPI_DEC_ALPHA;
+    const bool rmw_x = top && epi.mode == EPI_DEC_BETA;
+    const int er = top ? gr : gr - hn;  // row in eps and x
+    if (rmw_eps) {
+      double2 e[16], xv[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int gc = col_of(r);
-      if (gc < n) {
-        const double2 v = make_double2(cre[r], cim[r]);
-        if (gr < hn) {
-          const long long o = gr + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA) {
-            eps[o] = csub(eps[o], v);
-            xx[o] = csub(xx[o], v);
-          } else {
-            abn[gr + (long long)gc * 2 * hn] = v;
-            part += v.x * v.x + v.y * v.y;
-          }
-        } else {
-          const long long o = (gr - hn) + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA)
-            abn[gr + (long long)gc * 2 * hn] = v;
-          else
-            eps[o] = csub(eps[o], v);
+      for (int r = 0; r < 16; ++r) {
+        const long long o = er + (long long)min(col_of(r), n - 1) * hn;
+        e[r] = i8_gload(eps + o);
+        if (rmw_x) xv[r] = i8_gload(xx + o);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gc = col_of(r);
+        if (gc < n) {
+          const double2 v = make_double2(cre[r], cim[r]);
+          const long long o = er + (long long)gc * hn;
+          i8_gstore(eps + o, csub(e[r], v));
+          if (rmw_x) i8_gstore(xx + o, csub(xv[r], v));
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gc = col_of(r);
+        if (gc < n) {
+          const double2 v = make_double2(cre[r], cim[r]);
+          i8_gstore(abn + gr + (long long)gc * 2 * hn, v);
+          if (top) part += v.x * v.x + v.y * v.y;  // the top rows of alpha'
         }
       }
     }
