@@ -102,6 +102,19 @@ struct EpiArgs {
 //   p1 += ar*br, p2 += ai*bi, p3 += (ar+ai)*(br+bi);  re = p1 - p2,
 //   im = p3 - p1 - p2  (3/4 of the MFMA work; normwise FP64 accuracy).
 // ---------------------------------------------------------------------------
+// Global (not flat) loads and stores of a complex value: they count on the
+// vector-memory counter only, so a store waits for nothing that LDS does
+typedef double __attribute__((address_space(1))) gdbl_t;
+__device__ __forceinline__ double2 gload2(const double2* p) {
+  const gdbl_t* q = (const gdbl_t*)(const double*)p;
+  return make_double2(q[0], q[1]);
+}
+__device__ __forceinline__ void gstore2(double2* p, double2 v) {
+  gdbl_t* q = (gdbl_t*)(double*)p;
+  q[0] = v.x;
+  q[1] = v.y;
+}
+
 template <int TM, int TN, int KC, int WT = 2, bool M3 = false>
 __global__ __launch_bounds__((TM / 32) * (TN / (16 * WT)) * 64) void zgemm_f64_kernel(
     int m, int n, int k, const double2* const* __restrict__ Ap, int lda,

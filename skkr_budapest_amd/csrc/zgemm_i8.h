@@ -248,19 +248,6 @@ __device__ __forceinline__ void i8_chunk_products(const unsigned* __restrict__ p
   }
 }
 
-// Global (not flat) loads and stores of a complex value: they count on the
-// vector-memory counter only, so a store waits for nothing that LDS does
-typedef double __attribute__((address_space(1))) i8_gdbl;
-__device__ __forceinline__ double2 i8_gload(const double2* p) {
-  const i8_gdbl* q = (const i8_gdbl*)(const double*)p;
-  return make_double2(q[0], q[1]);
-}
-__device__ __forceinline__ void i8_gstore(double2* p, double2 v) {
-  i8_gdbl* q = (i8_gdbl*)(double*)p;
-  q[0] = v.x;
-  q[1] = v.y;
-}
-
 // Write the wave's 32 x 32 block (lane: C row row0 + rA; register r: column
 // col0 + 32 wn + 8 (r / 4) + 4 hf + r % 4) with the plain or the decimation
 // epilogue.  red: NT doubles of free LDS for the work-group's |alpha'|^2 sum.
@@ -280,8 +267,8 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
         if (gc < n) {
           double2 v = cmul(alpha, make_double2(cre[r], cim[r]));
           const long long o = gr + (long long)gc * ldc;
-          if (use_beta) v = cadd(v, cmul(beta, i8_gload(C + o)));
-          i8_gstore(C + o, v);
+          if (use_beta) v = cadd(v, cmul(beta, gload2(C + o)));
+          gstore2(C + o, v);
         }
       }
     }
@@ -307,8 +294,8 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const long long o = er + (long long)min(col_of(r), n - 1) * hn;
-        e[r] = i8_gload(eps + o);
-        if (rmw_x) xv[r] = i8_gload(xx + o);
+        e[r] = gload2(eps + o);
+        if (rmw_x) xv[r] = gload2(xx + o);
       }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -316,8 +303,8 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
         if (gc < n) {
           const double2 v = make_double2(cre[r], cim[r]);
           const long long o = er + (long long)gc * hn;
-          i8_gstore(eps + o, csub(e[r], v));
-          if (rmw_x) i8_gstore(xx + o, csub(xv[r], v));
+          gstore2(eps + o, csub(e[r], v));
+          if (rmw_x) gstore2(xx + o, csub(xv[r], v));
         }
       }
     } else {
@@ -326,7 +313,7 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
         const int gc = col_of(r);
         if (gc < n) {
           const double2 v = make_double2(cre[r], cim[r]);
-          i8_gstore(abn + gr + (long long)gc * 2 * hn, v);
+          gstore2(abn + gr + (long long)gc * 2 * hn, v);
           if (top) part += v.x * v.x + v.y * v.y;  // the top rows of alpha'
         }
       }

@@ -294,8 +294,8 @@ __global__ __launch_bounds__(512) void zgemm_split_kernel(
         if (gr < m && gc < n) {
           double2 r = cmul(alpha, make_double2(cre[bi][q], cim[bi][q]));
           const long long o = gr + (long long)gc * ldc;
-          if (use_beta) r = cadd(r, cmul(beta, C[o]));
-          C[o] = r;
+          if (use_beta) r = cadd(r, cmul(beta, gload2(C + o)));
+          gstore2(C + o, r);
         }
       }
     return;
@@ -305,31 +305,49 @@ __global__ __launch_bounds__(512) void zgemm_split_kernel(
   double2* __restrict__ xx = epi.x[b];
   double2* __restrict__ abn = epi.abn[b];
   double part = 0.0;
+  // Top rows (gr < hn): beta mode eps -= v and x -= v, alpha mode alpha' = v;
+  // bottom rows: beta mode beta' = v, alpha mode eps -= v.  The four values of
+  // a row issue all their read-modify-write loads before the first store (a
+  // column beyond n reads column n - 1 and stores nothing).
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < 2; ++bi) {
+    const int gr = row_of(bi);
+    if (gr < m) {
+      const bool top = gr < hn;
+      const bool rmw_eps = top ? epi.mode == EPI_DEC_BETA : epi.mode == EPI_DEC_ALPHA;
+      const bool rmw_x = top && epi.mode == EPI_DEC_BETA;
+      const int er = top ? gr : gr - hn;  // row in eps and x
+      if (rmw_eps) {
+        double2 e[4], xv[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int gr = row_of(bi), gc = col_of(q);
-      if (gr < m && gc < n) {
-        const double2 v = make_double2(cre[bi][q], cim[bi][q]);
-        if (gr < hn) {
-          const long long o = gr + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA) {
-            eps[o] = csub(eps[o], v);
-            xx[o] = csub(xx[o], v);
-          } else {
-            abn[gr + (long long)gc * 2 * hn] = v;
-            part += v.x * v.x + v.y * v.y;
+        for (int q = 0; q < 4; ++q) {
+          const long long o = er + (long long)min(col_of(q), n - 1) * hn;
+          e[q] = gload2(eps + o);
+          if (rmw_x) xv[q] = gload2(xx + o);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int gc = col_of(q);
+          if (gc < n) {
+            const double2 v = make_double2(cre[bi][q], cim[bi][q]);
+            const long long o = er + (long long)gc * hn;
+            gstore2(eps + o, csub(e[q], v));
+            if (rmw_x) gstore2(xx + o, csub(xv[q], v));
           }
-        } else {
-          const long long o = (gr - hn) + (long long)gc * hn;
-          if (epi.mode == EPI_DEC_BETA)
-            abn[gr + (long long)gc * 2 * hn] = v;
-          else
-            eps[o] = csub(eps[o], v);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int gc = col_of(q);
+          if (gc < n) {
+            const double2 v = make_double2(cre[bi][q], cim[bi][q]);
+            gstore2(abn + gr + (long long)gc * 2 * hn, v);
+            if (top) part += v.x * v.x + v.y * v.y;  // the top rows of alpha'
+          }
         }
       }
     }
+  }
   if (epi.mode == EPI_DEC_ALPHA) {
     // deterministic work-group sum of |alpha'|^2 (fixed-order tree in LDS,
     // in the slice planes, which are free now)
