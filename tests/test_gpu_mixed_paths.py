@@ -109,3 +109,28 @@ def test_int8_tiles_give_the_same_kloop(gpu):
     np.testing.assert_array_equal(a["iters"], b["iters"])
     np.testing.assert_array_equal(a["tau_bz"], b["tau_bz"])
     np.testing.assert_array_equal(a["trk"], b["trk"])
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("lk,s,P", [(9, 1, 3), (18, 2, 4), (24, 3, 3), (20, 5, 2), (26, 5, 2)])
+def test_mixed_kloop_ragged_shapes(gpu, mode, lk, s, P):
+    """The mixed k-loop on PL widths n = 9, 36, 72, 100, 130: partial tiles,
+    partial K-chunks, decimation products whose tiles straddle the ε / α'
+    halves (rows n - 1 | n), and n > 128 (the 2 x 2 block inversion); 9
+    k-points with an uneven last batch of 4 (batch 5).  Against the oracle."""
+    sy = S.make_system(S.small_config(lk=lk, s=s, P=P, r2max=4, tmodel="herm",
+                                      energies=(0.45 + 0.05j,), cfg_id=95))
+    E = sy.cfg.energies[0]
+    rng = np.random.default_rng(lk * 100 + s)
+    k = rng.uniform(-np.pi, np.pi, (9, 2))
+    w = np.full(9, 1.0 / 9)
+    ref = oracle.tau_bz(sy, E, k, w, nworkers=8)
+    print(f"{mode} ragged k-loop n={lk * s}", flush=True)
+    with gpu.Context.for_system(sy, precision=getattr(gpu, mode), batch=5) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.tau_bz(E, k, w)
+    assert (r["status"] == 0).all()
+    assert np.abs(r["iters"] - ref["iters"]).max() <= 1
+    e = rel(r["tau_bz"], ref["tau_bz"])
+    print("  tau rel. error", e, flush=True)
+    assert e < MIXED_TOL
