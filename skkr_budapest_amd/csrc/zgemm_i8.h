@@ -261,14 +261,20 @@ __device__ __forceinline__ void i8_epilogue(int m, int n, int row0, int col0, in
   if (epi.mode == EPI_PLAIN) {
     const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
     if (gr < m) {
+      // with beta, all 16 loads of C before the first store (a column beyond
+      // n reads column n - 1 and stores nothing)
+      double2 c0[16];
+      if (use_beta) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c0[r] = gload2(C + gr + (long long)min(col_of(r), n - 1) * ldc);
+      }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int gc = col_of(r);
         if (gc < n) {
           double2 v = cmul(alpha, make_double2(cre[r], cim[r]));
-          const long long o = gr + (long long)gc * ldc;
-          if (use_beta) v = cadd(v, cmul(beta, gload2(C + o)));
-          gstore2(C + o, v);
+          if (use_beta) v = cadd(v, cmul(beta, c0[r]));
+          gstore2(C + gr + (long long)gc * ldc, v);
         }
       }
     }

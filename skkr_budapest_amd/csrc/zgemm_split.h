@@ -286,6 +286,18 @@ __global__ __launch_bounds__(512) void zgemm_split_kernel(
   auto col_of = [&](int q) { return col0 + wn * 16 + 4 * l4 + q; };
   if (epi.mode == EPI_PLAIN) {
     const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
+    // with beta, the lane's loads of C before its first store (an element
+    // outside m x n reads element (0, 0) and stores nothing)
+    double2 c0[2][4];
+    if (use_beta) {
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int gr = row_of(bi), gc = col_of(q);
+          c0[bi][q] = gload2(C + ((gr < m && gc < n) ? gr + (long long)gc * ldc : 0ll));
+        }
+    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -293,9 +305,8 @@ __global__ __launch_bounds__(512) void zgemm_split_kernel(
         const int gr = row_of(bi), gc = col_of(q);
         if (gr < m && gc < n) {
           double2 r = cmul(alpha, make_double2(cre[bi][q], cim[bi][q]));
-          const long long o = gr + (long long)gc * ldc;
-          if (use_beta) r = cadd(r, cmul(beta, gload2(C + o)));
-          gstore2(C + o, r);
+          if (use_beta) r = cadd(r, cmul(beta, c0[bi][q]));
+          gstore2(C + gr + (long long)gc * ldc, r);
         }
       }
     return;
