@@ -58,11 +58,12 @@ enum {
                            decimation runs in FP64.  The per-routine entry
                            points stay FP64. */
   SKKR_MIXED_INT8 = 3   /* INT8 digit products: the same GEMMs as exact
-                           products of S = 6 balanced base-256 digits per
-                           operand value (2^-47 of its row / column maximum
-                           per K-chunk of 32) on the INT8 MFMA, summed in
-                           int32 and recombined in FP64 per chunk; the rest
-                           as SKKR_MIXED_BF16 */
+                           products of S balanced base-256 digits per
+                           operand value (S = 6: 2^-47 of its row / column
+                           maximum per K-chunk of 32; S = 5 in the sweeps
+                           above fp64_dec_below_imE: 2^-39) on the INT8
+                           MFMA, summed in int32 and recombined in FP64 per
+                           chunk; the rest as SKKR_MIXED_BF16 */
 };
 
 typedef struct skkr_ctx skkr_ctx;
@@ -186,7 +187,7 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *               operand (2..5, or 0 for FP64 products) of the decimation, of
  *               the sweeps, and of the sweeps below fp64_dec_below_imE
  *               (defaults 5, 5, 5); SKKR_MIXED_INT8: digits per operand
- *               (2..6, or 0; defaults 6, 6, 6)
+ *               (2..6, or 0; defaults 6, 5, 6)
  *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
  *               that many slices (0: FP64)
  *   "zgemm_int8" 0 | 2..6: skkr_zgemm_batch runs the INT8 digit product with

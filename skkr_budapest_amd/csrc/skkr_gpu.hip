@@ -995,9 +995,13 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
   skkr_ctx* c = new skkr_ctx();
   c->cfg = *cfg;
   if (cfg->precision == SKKR_MIXED_INT8) {
-    // 6 digits (2^-47 of the line maximum per operand value) in every phase
+    // 6 digits (2^-47 of the line maximum per operand value) in the
+    // decimation and in the sweeps near the real axis, 5 (2^-39) in the
+    // sweeps above fp64_dec_below_imE: the sweeps need ~36-40 bits, the
+    // decimation ~45 (config 1: tau within 3.7e-10 of FP64)
     c->split_i8 = 1;
-    c->split_dec = c->split_sw = c->split_sw_low = 6;
+    c->split_dec = c->split_sw_low = 6;
+    c->split_sw = 5;
   }
   if (const char* e = getenv("SKKR_INV_KERNEL"))
     c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
