@@ -61,7 +61,8 @@ enum {
                            products of S balanced base-256 digits per
                            operand value (S = 6: 2^-47 of its row / column
                            maximum per K-chunk of 32; S = 5 in the sweeps
-                           above fp64_dec_below_imE: 2^-39) on the INT8
+                           above fp64_dec_below_imE and in the decimation
+                           for Im E >= 0.1: 2^-39) on the INT8
                            MFMA, summed in int32 and recombined in FP64 per
                            chunk; the rest as SKKR_MIXED_BF16 */
 };
@@ -188,6 +189,8 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *               the sweeps, and of the sweeps below fp64_dec_below_imE
  *               (defaults 5, 5, 5); SKKR_MIXED_INT8: digits per operand
  *               (2..6, or 0; defaults 6, 5, 6)
+ *   "split_dec_high" the decimation's slices / digits for Im E >= 0.1 (0: as
+ *               split_dec; defaults: BF16 0, INT8 5)
  *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
  *               that many slices (0: FP64)
  *   "zgemm_int8" 0 | 2..6: skkr_zgemm_batch runs the INT8 digit product with

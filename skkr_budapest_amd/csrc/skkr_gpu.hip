@@ -39,6 +39,9 @@
     }                                                                      \
   } while (0)
 
+// Im E from which the decimation takes split_dec_high slices / digits
+constexpr double kDecHighImE = 0.1;
+
 // work space of skkr_surfgfd_noalloc, per GPU (64 MiB)
 #define SKKR_NOALLOC_SCRATCH_BYTES (64 << 20)
 
@@ -140,6 +143,7 @@ struct skkr_ctx {
   int split_i8 = 0;       // 1: INT8 digit products (zgemm_i8.h), 0: BF16 slices
   int i8_tile = 64;       // INT8 kernel: 64 (64 x 64 tiles, two per CU) or 128 (128 x 64, one per CU)
   int split_dec = 5;      // decimation and lead self-energies, Im E >= fp64_dec_below_imE
+  int split_dec_high = 0; // the same for Im E >= kDecHighImE (0: as split_dec)
   int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
   int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
   int zgemm_split = 0;    // skkr_zgemm_batch: 0 FP64, else slices or digits
@@ -1002,6 +1006,10 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
     c->split_i8 = 1;
     c->split_dec = c->split_sw_low = 6;
     c->split_sw = 5;
+    // far from the real axis the decimation converges in fewer iterations and
+    // amplifies less: 5 digits there (config-1 system: tau within 7.4e-10 of
+    // FP64 at Im E = 0.1, 1.5e-10 at 0.2)
+    c->split_dec_high = 5;
   }
   if (const char* e = getenv("SKKR_INV_KERNEL"))
     c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
@@ -1281,7 +1289,8 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   int s_dec = 0, s_sw = 0;
   if (c->cfg.precision == SKKR_MIXED_BF16 || c->cfg.precision == SKKR_MIXED_INT8) {
     const bool near_axis = c->e_im < c->cfg.fp64_dec_below_imE;
-    s_dec = near_axis ? 0 : c->split_dec;
+    const bool far = c->e_im >= kDecHighImE && c->split_dec_high > 0;
+    s_dec = near_axis ? 0 : far ? c->split_dec_high : c->split_dec;
     s_sw = near_axis ? c->split_sw_low : c->split_sw;
   }
   HIPCHK(hipMemsetAsync(c->taubz, 0, sizeof(z) * tsz, c->stream));
@@ -1569,11 +1578,13 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     drop_graphs(c);
     return SKKR_OK;
   }
-  if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0) {
+  if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0 ||
+      strcmp(key, "split_dec_high") == 0) {
     // BF16 slices 2..5, INT8 digits 2..6 (by the context's precision)
     const int smax = c->split_i8 ? 6 : 5;
     if (value != 0 && (value < 2 || value > smax)) return SKKR_ERR_ARG;
     if (strcmp(key, "split_dec") == 0) c->split_dec = value;
+    else if (strcmp(key, "split_dec_high") == 0) c->split_dec_high = value;
     else if (strcmp(key, "split_sw") == 0) c->split_sw = value;
     else c->split_sw_low = value;
     drop_graphs(c);

@@ -188,6 +188,9 @@ def test_mixed_precision_choices(gpu):
         d0 = c.tau_bz(E, k, w)
         with pytest.raises(gpu.SkkrError):
             c.set_option("split_sw", 7)
+        with pytest.raises(gpu.SkkrError):
+            c.set_option("split_dec_high", 7)
+        c.set_option("split_dec_high", 0)  # 0: the decimation takes split_dec at every Im E
     np.testing.assert_array_equal(d0["tau_bz"], ref["tau_bz"])
     f6 = O.rel_err(d6["tau_bz"], ref["tau_bz"])
     f3 = O.rel_err(d3["tau_bz"], ref["tau_bz"])
