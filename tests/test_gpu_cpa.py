@@ -141,3 +141,25 @@ def test_cpa_and_pairs_reject_bad_arguments(gpu):
         ctx._npair = 1
         with pytest.raises(gpu.SkkrError):   # no pairs set
             ctx.tau_bz_pairs(E, k, w)
+
+
+@pytest.mark.parametrize("mode", ["MIXED_INT8", "MIXED_BF16"])
+def test_cpa_loop_in_mixed_modes(gpu, mode):
+    """The CPA loop on a mixed-precision context (its k-loops with INT8-digit
+    or split-BF16 products) against the FP64 oracle's CPA loop: converged to
+    the same fixed point within 1e-8 relative (the mixed modes' bound), in at
+    most two CPA passes more."""
+    sy, E, conc, ta, tb = _alloy()
+    k, w = S.uniform_kmesh(6)
+    ref = oracle.cpa_bz(sy, E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10, nworkers=4)
+    print(f"cpa loop {mode}", flush=True)
+    with gpu.Context.for_system(sy, precision=getattr(gpu, mode)) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        r = ctx.cpa_bz(E, k, w, conc, ta, tb, itcpam=60, cpatol=1e-10)
+    print("  itcpa", r["itcpa"], "of", ref["itcpa"], "error", r["error"], flush=True)
+    assert r["error"] <= 1e-10
+    assert r["itcpa"] <= ref["itcpa"] + 2
+    for key in ("tau_c", "tau_a", "tau_b", "tinv_c"):
+        e = rel(r[key], ref[key])
+        print("  ", key, e, flush=True)
+        assert e < 1e-8, key
