@@ -134,3 +134,29 @@ def test_mixed_kloop_ragged_shapes(gpu, mode, lk, s, P):
     e = rel(r["tau_bz"], ref["tau_bz"])
     print("  tau rel. error", e, flush=True)
     assert e < MIXED_TOL
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_mixed_physical_tau(gpu, mode):
+    """skkr_tau_bz_phys in the mixed modes: the physical BZ sum and the per-k
+    per-l traces of phystau (cpacoord.f:294-311) against the oracle's
+    phystau of the FP64 oracle k-loop, within 1e-8."""
+    sy = S.make_system(S.small_config(lk=8, s=2, P=3, r2max=4, tmodel="herm", energies=(0.35 + 0.04j,)))
+    E = sy.cfg.energies[0]
+    k, w = S.uniform_kmesh(4)
+    nl, lk, ns = 3, 8, 6
+    rng = np.random.default_rng(8)
+    lidx = np.sort(np.concatenate([np.arange(nl), rng.integers(0, nl, size=lk - nl)]))
+    a = (0.2 * (rng.normal(size=(ns, nl)) + 1j * rng.normal(size=(ns, nl))))[:, lidx]
+    ref = oracle.tau_bz(sy, E, k, w, nworkers=8, want_tau_k=True)
+    tint = sy.tinv(E)[0]
+    print(f"{mode} physical tau", flush=True)
+    with gpu.Context.for_system(sy, precision=getattr(gpu, mode)) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        ctx.set_screening(a, lidx, nl)
+        r = ctx.tau_bz_phys(E, k, w)
+    phys_ref = np.stack([oracle.phystau(ref["tau_bz"][i], tint[i], tint[i], a[i]) for i in range(ns)])
+    assert rel(r["tau_bz_phys"], phys_ref) < MIXED_TOL
+    tkd_ref = np.array([[oracle.phys_traces(ref["tau_k"][ik, i], tint[i], a[i], lidx, nl)
+                         for i in range(ns)] for ik in range(len(w))])
+    assert rel(r["taukdiag"], tkd_ref) < MIXED_TOL
