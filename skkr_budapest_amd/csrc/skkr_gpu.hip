@@ -97,6 +97,10 @@ struct skkr_ctx {
   z** dv_out[4] = {nullptr, nullptr, nullptr, nullptr};  // out + offsets (B11, B21, B12, B22)
   int *active = nullptr, *iters = nullptr, *status = nullptr, *itermax = nullptr,
       *nactive = nullptr, *status_sw = nullptr, *active0 = nullptr;
+  // per-chain digit counts of the next decimation iteration (full / late) and
+  // the iterations a chain ran at the late count; |alpha'| after iteration 1
+  int *act_hi = nullptr, *act_lo = nullptr, *nlate = nullptr;
+  double* nref = nullptr;
   int dec_hint = 8;  // iterations launched before the first convergence read-back
   z *Gs = nullptr, *Zs = nullptr, *Ycur = nullptr, *W = nullptr, *Zb = nullptr,
     *Xc = nullptr, *tauk = nullptr, *trk = nullptr;
@@ -144,6 +148,9 @@ struct skkr_ctx {
   int i8_tile = 64;       // INT8 kernel: 64 (64 x 64 tiles, two per CU) or 128 (128 x 64, one per CU)
   int split_dec = 5;      // decimation and lead self-energies, Im E >= fp64_dec_below_imE
   int split_dec_high = 0; // the same for Im E >= kDecHighImE (0: as split_dec)
+  int split_dec_late = 0; // a chain's decimation iterations once |alpha'| fell below
+  int dec_late_log2 = 6;  //   2^-dec_late_log2 of its value after iteration 1 (0: off)
+  bool dom_nominal_off = false;  // the launch adds no nominal work to the roofline sums
   int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
   int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
   int zgemm_split = 0;    // skkr_zgemm_batch: 0 FP64, else slices or digits
@@ -282,6 +289,10 @@ static int ensure_batch(skkr_ctx* c, int B) {
   // still-active chain count after each decimation iteration (index = iteration)
   CHK(dalloc_bt(c, &c->nactive, (size_t)std::max(c->cfg.itermax_l, c->cfg.itermax_r) + 2));
   CHK(dalloc_bt(c, &c->active0, B2));
+  CHK(dalloc_bt(c, &c->act_hi, B2));
+  CHK(dalloc_bt(c, &c->act_lo, B2));
+  CHK(dalloc_bt(c, &c->nlate, B2));
+  CHK(dalloc_bt(c, &c->nref, B2));
   CHK(dalloc_bt(c, &c->status_sw, (size_t)B));
   CHK(dalloc_bt(c, &c->Gs, P * B * nn));
   CHK(dalloc_bt(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
@@ -492,9 +503,10 @@ static int launch_zgemm_i8(skkr_ctx* c, int m, int n, int k, z** A, int lda, z**
   const int ti = c->i8_tile;
   const double pm = (double)((m + ti - 1) / ti * ti), pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
   const double pairs = c->split * (c->split + 1) / 2.0;
-  if (c->timing_detail && !c->capturing) c->dom_flops_exec += pairs * 4.0 * 2.0 * pm * pn * pk * nb;
-  dom_end(c, 8.0 * m * n * (double)k * nb,
-          16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
+  const double nom = c->dom_nominal_off ? 0.0 : 1.0;
+  if (c->timing_detail && !c->capturing) c->dom_flops_exec += nom * pairs * 4.0 * 2.0 * pm * pn * pk * nb;
+  dom_end(c, nom * 8.0 * m * n * (double)k * nb,
+          nom * 16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
   c->launches++;
   return SKKR_OK;
 }
@@ -519,9 +531,10 @@ static int launch_zgemm_split(skkr_ctx* c, int m, int n, int k, z** A, int lda, 
   // the padded 64 x 64 x 32 tiles
   const double pm = (m + 63) / 64 * 64.0, pn = (n + 63) / 64 * 64.0, pk = (k + 31) / 32 * 32.0;
   const double pairs = c->split * (c->split + 1) / 2.0;
-  if (c->timing_detail && !c->capturing) c->dom_flops_exec += pairs * 4.0 * 2.0 * pm * pn * pk * nb;
-  dom_end(c, 8.0 * m * n * (double)k * nb,
-          16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
+  const double nom = c->dom_nominal_off ? 0.0 : 1.0;
+  if (c->timing_detail && !c->capturing) c->dom_flops_exec += nom * pairs * 4.0 * 2.0 * pm * pn * pk * nb;
+  dom_end(c, nom * 8.0 * m * n * (double)k * nb,
+          nom * 16.0 * ((double)m * k + (double)k * n + (double)m * n) * nb);
   c->launches++;
   return SKKR_OK;
 }
@@ -723,9 +736,38 @@ static int run_fourier(skkr_ctx* c, int nk) {
 // [ay; by] alpha with the updates of eps, x and the next [alpha; beta] fused
 // into the GEMM epilogues, and the per-chain stop test.  cur: the buffer of
 // [alpha; beta] that iteration `first` reads.
+// Whether the decimation products of the current digit count switch, per
+// chain, to split_dec_late digits late in the iteration (the products then
+// add a correction of relative size ~|alpha|^2 |y| / |eps| to eps and x).
+// Each product is then two launches from iteration 3 on; below n = 64 the
+// second launch costs more than the fewer digits save (n = 36, 512 chains:
+// the decimation 1.88 -> 1.96 ms per energy).
+constexpr int kDecLateMinN = 64;
+static bool dec_late_on(const skkr_ctx* c, int s) {
+  return s > 0 && c->split_dec_late > 0 && c->split_dec_late < s && c->dec_late_log2 > 0 && c->act_hi &&
+         c->n >= kDecLateMinN;
+}
+
 static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) {
   const int n = c->n, B2 = 2 * c->B;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  const bool late = dec_late_on(c, c->split);
+  const int* act = late ? c->act_hi : c->active;
+  // A product for the chains at the full digit count, then, from iteration 3
+  // (the first that a chain can run at the late count), for those at the
+  // late count; each chain is in exactly one of the two launches.
+  auto product = [&](int iter, int m, z** A, int lda, z** B, int ldb, z** C, int ldc,
+                     const EpiArgs* e) -> int {
+    CHK(launch_zgemm(c, m, n, n, A, lda, B, ldb, C, ldc, one, zero, B2, act, e));
+    if (late && iter >= 3) {
+      SplitScope sl(c, c->split_dec_late);
+      c->dom_nominal_off = true;
+      const int r = launch_zgemm(c, m, n, n, A, lda, B, ldb, C, ldc, one, zero, B2, c->act_lo, e);
+      c->dom_nominal_off = false;
+      CHK(r);
+    }
+    return SKKR_OK;
+  };
   for (int iter = first; iter <= last; ++iter) {
     InvArgs ia = inv_args(c);
     ia.base = c->p_eps.d;
@@ -733,8 +775,7 @@ static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) 
     ia.active = c->active;
     ia.status = c->status;
     CHK(run_inverse(c, ia, B2));
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n,
-                     one, zero, B2, c->active));
+    CHK(product(iter, 2 * n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n, nullptr));
     EpiArgs e;
     memset(&e, 0, sizeof(e));
     e.n = n;
@@ -744,14 +785,14 @@ static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) 
     e.norm = c->dnorm;
     e.tiles = tiles;
     e.mode = EPI_DEC_BETA;
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n,
-                     one, zero, B2, c->active, &e));
+    CHK(product(iter, 2 * n, c->p_AYBY.d, 2 * n, c->p_beta[cur].d, 2 * n, nullptr, 2 * n, &e));
     e.mode = EPI_DEC_ALPHA;
-    CHK(launch_zgemm(c, 2 * n, n, n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n,
-                     one, zero, B2, c->active, &e));
+    CHK(product(iter, 2 * n, c->p_AYBY.d, 2 * n, c->p_alpha[cur].d, 2 * n, nullptr, 2 * n, &e));
     dec_check_kernel<<<(B2 + 255) / 256, 256, 0, c->stream>>>(
         B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
-        c->active, c->iters, c->status, c->nactive + iter);
+        c->active, c->iters, c->status, c->nactive + iter,
+        late ? std::ldexp(1.0, -c->dec_late_log2) : 0.0, c->nref, late ? c->act_hi : nullptr,
+        c->act_lo, c->nlate);
     HIPCHK(hipGetLastError());
     c->launches++;
     cur ^= 1;
@@ -802,7 +843,9 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
   const int n = c->n, B2 = 2 * c->B;
   dec_init_kernel<<<B2, 256, 0, c->stream>>>(n, c->cfg.lk, c->p_ml.d, c->p_mr.d, c->p_m0.d,
                                              add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
-                                             c->p_eps.d, c->p_x.d);
+                                             c->p_eps.d, c->p_x.d, c->active,
+                                             dec_late_on(c, c->split) ? c->act_hi : nullptr,
+                                             c->act_lo, c->nlate);
   HIPCHK(hipGetLastError());
   // the chains started (restored for the final inversion and products)
   HIPCHK(hipMemcpyAsync(c->active0, c->active, B2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
@@ -1010,6 +1053,12 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
     // amplifies less: 5 digits there (config-1 system: tau within 7.4e-10 of
     // FP64 at Im E = 0.1, 1.5e-10 at 0.2)
     c->split_dec_high = 5;
+    // a chain's decimation products once |alpha'| is below 2^-6 of its value
+    // after iteration 1 add corrections ~|alpha|^2 |y| small against eps and
+    // x: 4 digits there (config-1 system at Im E 0.02 / 0.04 / 0.15: tau
+    // unchanged at 3.7e-10 / 1.8e-10 / 1.0e-10 of FP64, 1-3% less time)
+    c->split_dec_late = 4;
+    c->dec_late_log2 = 6;
   }
   if (const char* e = getenv("SKKR_INV_KERNEL"))
     c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
@@ -1300,7 +1349,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   double sum_l = 0, sum_r = 0;
   int maxit = 0, nbatches = 0;
   int rc = SKKR_OK;
-  std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B);
+  std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B), h_nl(2 * B, 0);
   std::vector<double> h_tr(2 * sites * B);
   std::vector<double> h_ph;
   // the CPA passes keep every k resident in one batch
@@ -1404,6 +1453,9 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     hipEventRecord(ev[4], c->stream);
     HIPCHK(hipMemcpyAsync(h_it.data(), c->iters, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(h_st.data(), c->status, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    const bool late_dec = !reuse && dec_late_on(c, s_dec);
+    if (c->timing_detail && late_dec)
+      HIPCHK(hipMemcpyAsync(h_nl.data(), c->nlate, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(h_sw.data(), c->status_sw, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (tr_tau_k)
       HIPCHK(hipMemcpyAsync(tr_tau_k + 2 * sites * (size_t)k0, c->trk, sizeof(z) * sites * nb,
@@ -1433,6 +1485,12 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
         const double pad = (double)((2 * c->n + tm - 1) / tm * tm) * ((c->n + 63) / 64 * 64) *
                            ((c->n + 31) / 32 * 32) / (2.0 * n3);
         c->dom_flops_exec += corr * s_dec * (s_dec + 1) / 2.0 * pad;
+        if (late_dec) {  // chain iterations at the late digit count (launches counted no nominal work)
+          double late_sum = 0.0;
+          for (int b = 0; b < nb; ++b) late_sum += h_nl[b] + h_nl[B + b];
+          const int sl = c->split_dec_late;
+          c->dom_flops_exec -= late_sum * 3.0 * 16.0 * n3 * (s_dec * (s_dec + 1) - sl * (sl + 1)) / 2.0 * pad;
+        }
       }
       c->dom_bytes += (it_sum - rounds * B2d) * 3.0 * 80.0 * n2 + (started - B2d) * 2.0 * 48.0 * n2;
       c->inv_flops += (it_sum - rounds * B2d) * 8.0 * n3 + (started - B2d) * 8.0 * n3;
@@ -1579,12 +1637,13 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     return SKKR_OK;
   }
   if (strcmp(key, "split_dec") == 0 || strcmp(key, "split_sw") == 0 || strcmp(key, "split_sw_low") == 0 ||
-      strcmp(key, "split_dec_high") == 0) {
+      strcmp(key, "split_dec_high") == 0 || strcmp(key, "split_dec_late") == 0) {
     // BF16 slices 2..5, INT8 digits 2..6 (by the context's precision)
     const int smax = c->split_i8 ? 6 : 5;
     if (value != 0 && (value < 2 || value > smax)) return SKKR_ERR_ARG;
     if (strcmp(key, "split_dec") == 0) c->split_dec = value;
     else if (strcmp(key, "split_dec_high") == 0) c->split_dec_high = value;
+    else if (strcmp(key, "split_dec_late") == 0) c->split_dec_late = value;
     else if (strcmp(key, "split_sw") == 0) c->split_sw = value;
     else c->split_sw_low = value;
     drop_graphs(c);
@@ -1595,6 +1654,12 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     if (value != 0 && (value < 2 || value > (i8 ? 6 : 5))) return SKKR_ERR_ARG;
     c->zgemm_split = value;
     c->zgemm_i8 = i8 ? 1 : 0;
+    return SKKR_OK;
+  }
+  if (strcmp(key, "dec_late_log2") == 0) {
+    if (value < 0 || value > 40) return SKKR_ERR_ARG;
+    c->dec_late_log2 = value;
+    drop_graphs(c);
     return SKKR_OK;
   }
   if (strcmp(key, "int8_tile") == 0) {

@@ -111,6 +111,48 @@ def test_mixed_config1_sample_matches_oracle(gpu, mode):
     assert np.abs(r["iters"] - ref["iters"]).max() <= 1
 
 
+def test_int8_late_decimation_digits(gpu):
+    """The INT8 decimation's per-chain late digits (4 once |alpha'| fell to
+    2^-6 of its value after iteration 1) on 32 k-points of config 1: with and
+    without them against the FP64 oracle, the same iteration counts, less
+    executed INT8 work for the same algorithmic work, and per-k results that
+    do not depend on the other chains of the batch (the rule reads only the
+    chain's own |alpha'|)."""
+    cfg = S.CONFIGS[1]
+    sy = S.make_system(cfg)
+    E = cfg.energies[0]
+    k, w = S.uniform_kmesh(cfg.mesh)
+    idx = np.linspace(0, len(w) - 1, 32).astype(int)
+    ks, ws = k[idx], np.full(len(idx), 1.0 / len(idx))
+    print("INT8 late decimation digits, cfg1 sample: 32 k", flush=True)
+    ref = oracle.tau_bz(sy, E, ks, ws, nworkers=16)
+    out = {}
+    with gpu.Context.for_system(sy, precision=gpu.MIXED_INT8) as ctx:
+        ctx.set_tinv(*sy.tinv(E))
+        for late in (4, 0):
+            ctx.set_option("split_dec_late", late)
+            out[late] = ctx.tau_bz(E, ks, ws, timing_detail=True)
+        ctx.set_option("split_dec_late", 4)
+        part = ctx.tau_bz(E, ks[:8], ws[:8])
+        for key, bad in (("split_dec_late", 7), ("split_dec_late", 1), ("dec_late_log2", 41),
+                         ("dec_late_log2", -1)):
+            with pytest.raises(gpu.SkkrError):
+                ctx.set_option(key, bad)
+    on, off = out[4], out[0]
+    e_on = _errors(on["tau_bz"], ref["tau_bz"])
+    e_off = _errors(off["tau_bz"], ref["tau_bz"])
+    print("late digits", e_on, "none", e_off, flush=True)
+    for v in list(e_on.values()) + list(e_off.values()):
+        assert v < MIXED_TOL
+    # the late products change eps and x by corrections far below the rest
+    assert e_on["tau_rel_err"] < 2.0 * e_off["tau_rel_err"] + 1e-10
+    np.testing.assert_array_equal(on["iters"], off["iters"])
+    assert on["stats"]["dom_flops"] == pytest.approx(off["stats"]["dom_flops"], rel=1e-12)
+    assert on["stats"]["dom_flops_exec"] < 0.97 * off["stats"]["dom_flops_exec"]
+    np.testing.assert_array_equal(part["trk"], on["trk"][:8])
+    np.testing.assert_array_equal(part["iters"], on["iters"][:8])
+
+
 def test_mixed_contour_config2(gpu):
     """Config 2: the config-1 system on the 16-point contour (Im E from 0.695
     down to 0.005), full 64 x 64 mesh per energy, both mixed modes (split-BF16
