@@ -64,8 +64,9 @@ enum {
                            above fp64_dec_below_imE and in the decimation
                            for Im E >= 0.1: 2^-39; S = 4 in a decimation
                            chain's late iterations, once |alpha'| fell to
-                           2^-6 of its value after iteration 1, for n >= 64)
-                           on the INT8
+                           2^-6 of its value after iteration 1, for n >= 64;
+                           from 2^-20 on the chain keeps eps^-1 of the
+                           iteration before) on the INT8
                            MFMA, summed in int32 and recombined in FP64 per
                            chunk; the rest as SKKR_MIXED_BF16 */
 };
@@ -196,9 +197,14 @@ int skkr_tau_bz_phys(skkr_ctx *ctx, double e_re, double e_im, int nk,
  *               split_dec; defaults: BF16 0, INT8 5)
  *   "split_dec_late", "dec_late_log2"  slices / digits (2..6, or 0: off) of a
  *               decimation chain's products from the iteration after which
- *               |alpha'|_F is at most 2^-dec_late_log2 (0..40; 0: off) of its
+ *               |alpha'|_F is at most 2^-dec_late_log2 (0..60; 0: off) of its
  *               value after iteration 1, when fewer than the decimation's
  *               and n >= 64 (defaults: BF16 0, INT8 4 and 6)
+ *   "dec_keep_y_log2" 0..60: a decimation chain's iterations keep eps^-1 of
+ *               the iteration before once |alpha'|_F is at most 2^-value of
+ *               its value after iteration 1 (eps then changes by far less
+ *               than the products' error; mixed modes only, 0: off;
+ *               defaults: BF16 0, INT8 20)
  *   "zgemm_split" 0 | 2..5: skkr_zgemm_batch runs the split-BF16 product with
  *               that many slices (0: FP64)
  *   "zgemm_int8" 0 | 2..6: skkr_zgemm_batch runs the INT8 digit product with

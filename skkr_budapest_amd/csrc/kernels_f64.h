@@ -529,12 +529,16 @@ __global__ __launch_bounds__(256) void dec_init_kernel(
     int n, int lk, const double2* const* ml, const double2* const* mr,
     const double2* const* m0, const double2* const* tinv, double2* const* AB,
     double2* const* eps, double2* const* x, const int* active, int* act_hi, int* act_lo,
-    int* nlate) {
+    int* nlate, int* act_inv, int* nkeep) {
   const int b = blockIdx.x;
   if (act_hi && threadIdx.x == 0) {  // every started chain begins at the full digit count
     act_hi[b] = active[b];
     act_lo[b] = 0;
     nlate[b] = 0;
+  }
+  if (act_inv && threadIdx.x == 0) {
+    act_inv[b] = active[b];
+    nkeep[b] = 0;
   }
   const double2* __restrict__ L = ml[b];
   const double2* __restrict__ Rm = mr[b];
@@ -557,10 +561,13 @@ __global__ __launch_bounds__(256) void dec_init_kernel(
 // b < nhalf are left leads (itermax_l), the others right leads (itermax_r).
 // With act_hi, the digit count of the chain's next iteration: the reduced
 // one (act_lo) once err <= late_ratio times its err after iteration 1 (nref).
+// With act_inv, whether the next iteration inverts eps: not once err <=
+// inv_ratio times nref (the iteration then keeps the previous eps^-1).
 __global__ __launch_bounds__(256) void dec_check_kernel(
     int nchains, int tiles, int iter, int itermax_l, int itermax_r, int nhalf, double errmax,
     const double* __restrict__ norm, int* active, int* iters, int* status, int* n_active,
-    double late_ratio, double* nref, int* act_hi, int* act_lo, int* nlate) {
+    double late_ratio, double* nref, int* act_hi, int* act_lo, int* nlate, double inv_ratio,
+    int* act_inv, int* nkeep) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b >= nchains || !active[b]) return;
   double s = 0.0;
@@ -577,16 +584,22 @@ __global__ __launch_bounds__(256) void dec_check_kernel(
     status[b] |= 2;
   } else {
     atomicAdd(n_active, 1);
+    if (iter == 1 && nref) nref[b] = err;
     if (act_hi) {
-      if (iter == 1) nref[b] = err;
       const int lo = err <= late_ratio * nref[b];
       act_hi[b] = !lo;
       act_lo[b] = lo;
       nlate[b] += lo;
     }
+    if (act_inv) {
+      const int keep = err <= inv_ratio * nref[b];
+      act_inv[b] = !keep;
+      nkeep[b] += keep;
+    }
     return;
   }
   if (act_hi) act_hi[b] = act_lo[b] = 0;
+  if (act_inv) act_inv[b] = 0;
 }
 
 // Assembly a = base + blockdiag(tinv) - sub1 - sub2 into a dense n x n buffer

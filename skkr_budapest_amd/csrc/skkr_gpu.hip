@@ -101,6 +101,8 @@ struct skkr_ctx {
   // the iterations a chain ran at the late count; |alpha'| after iteration 1
   int *act_hi = nullptr, *act_lo = nullptr, *nlate = nullptr;
   double* nref = nullptr;
+  int* act_inv = nullptr;  // chains whose next decimation iteration inverts eps
+  int* nkeep = nullptr;    // and the iterations that kept the previous inverse
   int dec_hint = 8;  // iterations launched before the first convergence read-back
   z *Gs = nullptr, *Zs = nullptr, *Ycur = nullptr, *W = nullptr, *Zb = nullptr,
     *Xc = nullptr, *tauk = nullptr, *trk = nullptr;
@@ -150,6 +152,8 @@ struct skkr_ctx {
   int split_dec_high = 0; // the same for Im E >= kDecHighImE (0: as split_dec)
   int split_dec_late = 0; // a chain's decimation iterations once |alpha'| fell below
   int dec_late_log2 = 6;  //   2^-dec_late_log2 of its value after iteration 1 (0: off)
+  int dec_keep_y_log2 = 0; // a chain's iterations keep eps^-1 once |alpha'| fell below
+                           //   2^-dec_keep_y_log2 of its value after iteration 1 (0: off)
   bool dom_nominal_off = false;  // the launch adds no nominal work to the roofline sums
   int split_sw = 5;       // PL sweeps, Im E >= fp64_dec_below_imE
   int split_sw_low = 5;   // PL sweeps below it (the decimation is FP64 there)
@@ -293,6 +297,8 @@ static int ensure_batch(skkr_ctx* c, int B) {
   CHK(dalloc_bt(c, &c->act_lo, B2));
   CHK(dalloc_bt(c, &c->nlate, B2));
   CHK(dalloc_bt(c, &c->nref, B2));
+  CHK(dalloc_bt(c, &c->act_inv, B2));
+  CHK(dalloc_bt(c, &c->nkeep, B2));
   CHK(dalloc_bt(c, &c->status_sw, (size_t)B));
   CHK(dalloc_bt(c, &c->Gs, P * B * nn));
   CHK(dalloc_bt(c, &c->Zs, (P > 1 ? P - 1 : 1) * B * nn));
@@ -747,11 +753,19 @@ static bool dec_late_on(const skkr_ctx* c, int s) {
   return s > 0 && c->split_dec_late > 0 && c->split_dec_late < s && c->dec_late_log2 > 0 && c->act_hi &&
          c->n >= kDecLateMinN;
 }
+// Whether a chain's late decimation iterations keep eps^-1 of the iteration
+// before: once |alpha'| <= 2^-dec_keep_y_log2 of its first value, eps changed
+// by a relative ~u in the last iteration and changes by ~u^2 in the next, so
+// the stale inverse puts an error ~u^3 into eps and x (mixed modes only).
+static bool dec_keep_y_on(const skkr_ctx* c, int s) {
+  return s > 0 && c->dec_keep_y_log2 > 0 && c->act_inv;
+}
 
 static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) {
   const int n = c->n, B2 = 2 * c->B;
   const z one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   const bool late = dec_late_on(c, c->split);
+  const bool keep_y = dec_keep_y_on(c, c->split);
   const int* act = late ? c->act_hi : c->active;
   // A product for the chains at the full digit count, then, from iteration 3
   // (the first that a chain can run at the late count), for those at the
@@ -772,7 +786,7 @@ static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) 
     InvArgs ia = inv_args(c);
     ia.base = c->p_eps.d;
     ia.out = c->p_Yinv.d;
-    ia.active = c->active;
+    ia.active = keep_y ? c->act_inv : c->active;
     ia.status = c->status;
     CHK(run_inverse(c, ia, B2));
     CHK(product(iter, 2 * n, c->p_AB[cur].d, 2 * n, c->p_Yinv.d, n, c->p_AYBY.d, 2 * n, nullptr));
@@ -792,7 +806,8 @@ static int dec_iterations(skkr_ctx* c, int first, int last, int cur, int tiles) 
         B2, tiles, iter, c->cfg.itermax_l, c->cfg.itermax_r, c->B, c->cfg.errmax, c->dnorm,
         c->active, c->iters, c->status, c->nactive + iter,
         late ? std::ldexp(1.0, -c->dec_late_log2) : 0.0, c->nref, late ? c->act_hi : nullptr,
-        c->act_lo, c->nlate);
+        c->act_lo, c->nlate, keep_y ? std::ldexp(1.0, -c->dec_keep_y_log2) : 0.0,
+        keep_y ? c->act_inv : nullptr, c->nkeep);
     HIPCHK(hipGetLastError());
     c->launches++;
     cur ^= 1;
@@ -845,7 +860,9 @@ static int run_decimation(skkr_ctx* c, bool add_tinv, bool want_sigma, const Ptr
                                              add_tinv ? c->p_tl.d : nullptr, c->p_AB[0].d,
                                              c->p_eps.d, c->p_x.d, c->active,
                                              dec_late_on(c, c->split) ? c->act_hi : nullptr,
-                                             c->act_lo, c->nlate);
+                                             c->act_lo, c->nlate,
+                                             dec_keep_y_on(c, c->split) ? c->act_inv : nullptr,
+                                             c->nkeep);
   HIPCHK(hipGetLastError());
   // the chains started (restored for the final inversion and products)
   HIPCHK(hipMemcpyAsync(c->active0, c->active, B2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
@@ -1059,6 +1076,10 @@ int skkr_create(skkr_ctx** out, const skkr_config* cfg) {
     // unchanged at 3.7e-10 / 1.8e-10 / 1.0e-10 of FP64, 1-3% less time)
     c->split_dec_late = 4;
     c->dec_late_log2 = 6;
+    // and from 2^-20 on it keeps eps^-1 of the iteration before (the same
+    // three energies: tau within 3.7e-10 / 1.8e-10 / 1.0e-10, 0.4-1.5% less
+    // time)
+    c->dec_keep_y_log2 = 20;
   }
   if (const char* e = getenv("SKKR_INV_KERNEL"))
     c->inv_kernel = (strcmp(e, "rowwise") == 0) ? 1 : (strcmp(e, "nopivot") == 0) ? 2 : 0;
@@ -1349,7 +1370,7 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
   double sum_l = 0, sum_r = 0;
   int maxit = 0, nbatches = 0;
   int rc = SKKR_OK;
-  std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B), h_nl(2 * B, 0);
+  std::vector<int> h_it(2 * B), h_st(2 * B), h_sw(B), act(2 * B), h_nl(2 * B, 0), h_nk(2 * B, 0);
   std::vector<double> h_tr(2 * sites * B);
   std::vector<double> h_ph;
   // the CPA passes keep every k resident in one batch
@@ -1456,6 +1477,9 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
     const bool late_dec = !reuse && dec_late_on(c, s_dec);
     if (c->timing_detail && late_dec)
       HIPCHK(hipMemcpyAsync(h_nl.data(), c->nlate, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    const bool keep_dec = !reuse && dec_keep_y_on(c, s_dec);
+    if (c->timing_detail && keep_dec)
+      HIPCHK(hipMemcpyAsync(h_nk.data(), c->nkeep, 2 * B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(h_sw.data(), c->status_sw, B * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if (tr_tau_k)
       HIPCHK(hipMemcpyAsync(tr_tau_k + 2 * sites * (size_t)k0, c->trk, sizeof(z) * sites * nb,
@@ -1493,8 +1517,10 @@ static int tau_bz_run(skkr_ctx* c, int nk, const double* k, const double* w, dou
         }
       }
       c->dom_bytes += (it_sum - rounds * B2d) * 3.0 * 80.0 * n2 + (started - B2d) * 2.0 * 48.0 * n2;
-      c->inv_flops += (it_sum - rounds * B2d) * 8.0 * n3 + (started - B2d) * 8.0 * n3;
-      c->inv_bytes += (it_sum - rounds * B2d) * 32.0 * n2 + (started - B2d) * 32.0 * n2;
+      double kept = 0.0;  // iterations that kept the previous inverse
+      for (int b = 0; b < nb; ++b) kept += h_nk[b] + h_nk[B + b];
+      c->inv_flops += (it_sum - kept - rounds * B2d) * 8.0 * n3 + (started - B2d) * 8.0 * n3;
+      c->inv_bytes += (it_sum - kept - rounds * B2d) * 32.0 * n2 + (started - B2d) * 32.0 * n2;
     }
     for (int b = 0; b < nb; ++b) {
       int il = h_it[b], ir = h_it[B + b];
@@ -1656,9 +1682,10 @@ int skkr_set_option(skkr_ctx* c, const char* key, int value) {
     c->zgemm_i8 = i8 ? 1 : 0;
     return SKKR_OK;
   }
-  if (strcmp(key, "dec_late_log2") == 0) {
-    if (value < 0 || value > 40) return SKKR_ERR_ARG;
-    c->dec_late_log2 = value;
+  if (strcmp(key, "dec_late_log2") == 0 || strcmp(key, "dec_keep_y_log2") == 0) {
+    if (value < 0 || value > 60) return SKKR_ERR_ARG;
+    if (strcmp(key, "dec_late_log2") == 0) c->dec_late_log2 = value;
+    else c->dec_keep_y_log2 = value;
     drop_graphs(c);
     return SKKR_OK;
   }

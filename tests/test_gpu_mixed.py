@@ -112,12 +112,13 @@ def test_mixed_config1_sample_matches_oracle(gpu, mode):
 
 
 def test_int8_late_decimation_digits(gpu):
-    """The INT8 decimation's per-chain late digits (4 once |alpha'| fell to
-    2^-6 of its value after iteration 1) on 32 k-points of config 1: with and
-    without them against the FP64 oracle, the same iteration counts, less
-    executed INT8 work for the same algorithmic work, and per-k results that
-    do not depend on the other chains of the batch (the rule reads only the
-    chain's own |alpha'|)."""
+    """The INT8 decimation's per-chain late iterations (4 digits once |alpha'|
+    fell to 2^-6 of its value after iteration 1, the previous eps^-1 kept from
+    2^-20 on) on 32 k-points of config 1: with and without them against the
+    FP64 oracle, the same iteration counts, less executed INT8 work and fewer
+    inversion flops for the same GEMM work, and per-k results that do not
+    depend on the other chains of the batch (the rules read only the chain's
+    own |alpha'|)."""
     cfg = S.CONFIGS[1]
     sy = S.make_system(cfg)
     E = cfg.energies[0]
@@ -131,11 +132,13 @@ def test_int8_late_decimation_digits(gpu):
         ctx.set_tinv(*sy.tinv(E))
         for late in (4, 0):
             ctx.set_option("split_dec_late", late)
+            ctx.set_option("dec_keep_y_log2", 20 if late else 0)
             out[late] = ctx.tau_bz(E, ks, ws, timing_detail=True)
         ctx.set_option("split_dec_late", 4)
+        ctx.set_option("dec_keep_y_log2", 20)
         part = ctx.tau_bz(E, ks[:8], ws[:8])
-        for key, bad in (("split_dec_late", 7), ("split_dec_late", 1), ("dec_late_log2", 41),
-                         ("dec_late_log2", -1)):
+        for key, bad in (("split_dec_late", 7), ("split_dec_late", 1), ("dec_late_log2", 61),
+                         ("dec_late_log2", -1), ("dec_keep_y_log2", 61)):
             with pytest.raises(gpu.SkkrError):
                 ctx.set_option(key, bad)
     on, off = out[4], out[0]
@@ -149,6 +152,7 @@ def test_int8_late_decimation_digits(gpu):
     np.testing.assert_array_equal(on["iters"], off["iters"])
     assert on["stats"]["dom_flops"] == pytest.approx(off["stats"]["dom_flops"], rel=1e-12)
     assert on["stats"]["dom_flops_exec"] < 0.97 * off["stats"]["dom_flops_exec"]
+    assert on["stats"]["inv_flops"] < 0.99 * off["stats"]["inv_flops"]
     np.testing.assert_array_equal(part["trk"], on["trk"][:8])
     np.testing.assert_array_equal(part["iters"], on["iters"][:8])
 
