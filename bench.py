@@ -413,7 +413,8 @@ def main():
         # the library's defaults for SKKR_MIXED_INT8 (skkr_create)
         cfg_out["mixed_int8_digits"] = ("decimation 6 (5 from Im E = 0.1, FP64 below fp64_dec_below_imE; "
                                         "4 in a chain's iterations after |alpha'| fell to 2^-6 of its first "
-                                        "value, for n >= 64), sweeps 5 (6 below fp64_dec_below_imE)")
+                                        "value, for n >= 64; eps^-1 of the iteration before kept from 2^-20 on), "
+                                        "sweeps 5 (6 below fp64_dec_below_imE)")
     if "mixed_bf16" in runs and "fp64" in runs:
         cfg_out["mixed_bf16_over_fp64"] = runs["fp64"]["ms"] / runs["mixed_bf16"]["ms"]
         cfg_out["mixed_bf16_error_vs_fp64"] = O.error_report(runs["mixed_bf16"]["tau_bz"],
